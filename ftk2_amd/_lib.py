@@ -1,0 +1,91 @@
+"""ctypes bindings of libftk2h.so (include/ftk2h.h).
+
+The library is loaded from the package directory only; there is no fallback: if
+it is missing, or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libftk2h.so")
+
+FTK2H_OK = 0
+STATUS = {0: "OK", 1: "E_INVALID", 2: "E_UNSUPPORTED", 3: "E_OVERFLOW", 4: "E_OOM", 5: "E_HIP"}
+PRED_CRITICAL_POINT, PRED_CONTOUR, PRED_FIBER = 0, 1, 2
+F32, F64 = 0, 1
+
+
+class Ftk2hError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("ndims", C.c_int), ("local_dims", C.c_uint64 * 4), ("offset", C.c_uint64 * 4),
+                ("global_dims", C.c_uint64 * 4)]
+
+
+class PredicateDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("codimension", C.c_int), ("dtype", C.c_int), ("sos_q", C.c_double),
+                ("thresholds", C.c_double * 2)]
+
+
+class FieldView(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("strides", C.c_int64 * 4), ("on_device", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("node_capacity", C.c_uint64), ("cell_capacity", C.c_uint64), ("sync", C.c_int)]
+
+
+class DeviceView(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint64), ("node_dim", C.c_int), ("keys", C.c_void_p),
+                ("lambda_", C.c_void_p), ("track_id", C.c_void_p), ("scalar", C.c_void_p),
+                ("n_conn", C.c_int), ("conn_dim", C.c_int * 3), ("conn_cells", C.c_uint64 * 3),
+                ("conn", C.c_void_p * 3)]
+
+
+# Every entry point of include/ftk2h.h: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "ftk2h_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "ftk2h_ctx_destroy": (None, [_P]),
+    "ftk2h_extract_regular": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(PredicateDesc),
+                                        C.POINTER(FieldView), C.c_int, C.POINTER(FieldView),
+                                        C.POINTER(Options), C.POINTER(_P), _P]),
+    "ftk2h_result_sizes": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ftk2h_result_conn_info": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "ftk2h_result_copy_nodes": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "ftk2h_result_copy_keys": (C.c_int, [_P, _P]),
+    "ftk2h_result_copy_conn": (C.c_int, [_P, C.c_int, _P]),
+    "ftk2h_result_device_view": (C.c_int, [_P, C.POINTER(DeviceView)]),
+    "ftk2h_result_phase_ms": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
+    "ftk2h_result_free": (None, [_P]),
+    "ftk2h_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libftk2h.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ftk2_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != FTK2H_OK:
+        raise Ftk2hError(code, lib().ftk2h_last_error().decode(errors="replace"))

@@ -1,0 +1,184 @@
+// Exact sign tests and the barycentric solver, device side.
+//
+// Semantics follow the reference CPU code bit for bit:
+//   quantization      include/ftk2/core/sos.hpp:15,25,75  (round half away from zero
+//                     after a double multiply and a double add, then truncate)
+//   sign              sos.hpp:12-20   (zero -> parity of the vertex id)
+//   det2              sos.hpp:22-35   (zero -> +1 if i0 < i1)
+//   det3              sos.hpp:72-200  (15 levels; index arguments unused; level 14 = +1)
+//   origin_inside<K>  sos.hpp:202-254
+//   solver            zero_crossing.hpp:20-105 (first-max partial pivoting,
+//                     |pivot| < 1e-12 fails, lambda in [-1e-9, 1+1e-9])
+// The library is compiled with -ffp-contract=off so no product is fused into a
+// following add; fused operations appear only where written as fma() and only in
+// exact-by-construction integer arithmetic or in error-bounded filters.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ftk2h {
+
+using i128 = __int128;
+
+template <class T>
+__device__ __forceinline__ double qd(T v, double Q) {
+  // double(v) * Q + (v > 0 ? 0.5 : -0.5), truncated toward zero later
+  const double prod = __dmul_rn((double)v, Q);
+  return __dadd_rn(prod, v > T(0) ? 0.5 : -0.5);
+}
+
+template <class T>
+__device__ __forceinline__ int64_t q64(T v, double Q) {
+  return (int64_t)qd(v, Q);  // truncation; inputs with |q| >= 2^62 are out of domain
+}
+
+__device__ __forceinline__ int sgn128(i128 x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+
+// det of [[a0,a1,1],[b0,b1,1],[c0,c1,1]]
+__device__ __forceinline__ i128 det3_ones(i128 a0, i128 a1, i128 b0, i128 b1, i128 c0, i128 c1) {
+  return a0 * (b1 - c1) - a1 * (b0 - c0) + (b0 * c1 - b1 * c0);
+}
+
+// The reference det3 (sos.hpp:72-200) on quantized rows X0, X1, X2; X3 = origin.
+// Level 0 (the 4x4 determinant with the ones column) equals det[X0; X1; X2].
+__device__ __noinline__ int det3_levels(const int64_t* X0, const int64_t* X1, const int64_t* X2) {
+  const i128 a0 = X0[0], a1 = X0[1], a2 = X0[2];
+  const i128 b0 = X1[0], b1 = X1[1], b2 = X1[2];
+  const i128 c0 = X2[0], c1 = X2[1], c2 = X2[2];
+  i128 d;
+  d = a0 * (b1 * c2 - b2 * c1) - a1 * (b0 * c2 - b2 * c0) + a2 * (b0 * c1 - b1 * c0);  // t=0
+  if (d != 0) return sgn128(d);
+  d = det3_ones(b0, b1, c0, c1, 0, 0);  // t=1: rows X1,X2,X3 cols x,y
+  if (d != 0) return sgn128(d);
+  d = -det3_ones(b0, b2, c0, c2, 0, 0);  // t=2: cols x,z, negated
+  if (d != 0) return sgn128(d);
+  d = det3_ones(b1, b2, c1, c2, 0, 0);  // t=3: cols y,z
+  if (d != 0) return sgn128(d);
+  d = -det3_ones(a0, a1, c0, c1, 0, 0);  // t=4: rows X0,X2,X3 cols x,y, negated
+  if (d != 0) return sgn128(d);
+  d = c0;  // t=5: [[X2x,1],[X3x,1]]
+  if (d != 0) return sgn128(d);
+  d = -c1;  // t=6
+  if (d != 0) return sgn128(d);
+  d = det3_ones(a0, a2, c0, c2, 0, 0);  // t=7: rows X0,X2,X3 cols x,z
+  if (d != 0) return sgn128(d);
+  d = c2;  // t=8
+  if (d != 0) return sgn128(d);
+  d = -det3_ones(a1, a2, c1, c2, 0, 0);  // t=9: rows X0,X2,X3 cols y,z, negated
+  if (d != 0) return sgn128(d);
+  d = det3_ones(a0, a1, b0, b1, 0, 0);  // t=10: rows X0,X1,X3 cols x,y
+  if (d != 0) return sgn128(d);
+  d = -b0;  // t=11
+  if (d != 0) return sgn128(d);
+  d = b1;  // t=12
+  if (d != 0) return sgn128(d);
+  d = a0;  // t=13
+  if (d != 0) return sgn128(d);
+  return 1;  // t=14
+}
+
+// origin_inside<3> (sos.hpp:234-254) on quantized vertices in ascending id order.
+__device__ __noinline__ bool inside3_exact(const int64_t (*X)[3]) {
+  const int d0 = det3_levels(X[1], X[3], X[2]);
+  const int d1 = det3_levels(X[0], X[2], X[3]);
+  const int d2 = det3_levels(X[0], X[3], X[1]);
+  const int d3 = det3_levels(X[0], X[1], X[2]);
+  return d0 == d1 && d1 == d2 && d2 == d3;
+}
+
+// det2 (sos.hpp:22-35) for ids i0 < i1 (the tie then returns +1) or i0 > i1 (-1).
+__device__ __forceinline__ int det2_exact(int64_t a, int64_t b, int64_t c, int64_t d, bool i0_lt_i1) {
+  const i128 det = (i128)a * d - (i128)b * c;
+  if (det > 0) return 1;
+  if (det < 0) return -1;
+  return i0_lt_i1 ? 1 : -1;
+}
+
+// origin_inside<2> (sos.hpp:222-232), vertices in ascending id order.
+__device__ __forceinline__ bool inside2_exact(const int64_t (*X)[2]) {
+  const int s0 = det2_exact(X[0][0], X[0][1], X[1][0], X[1][1], true);
+  const int s1 = det2_exact(X[1][0], X[1][1], X[2][0], X[2][1], true);
+  const int s2 = det2_exact(X[2][0], X[2][1], X[0][0], X[0][1], false);
+  return s0 == s1 && s1 == s2;
+}
+
+// sign (sos.hpp:12-20)
+template <class T>
+__device__ __forceinline__ int sos_sign(T v, uint64_t id, double Q) {
+  const int64_t q = q64(v, Q);
+  if (q > 0) return 1;
+  if (q < 0) return -1;
+  return (id % 2 == 0) ? 1 : -1;
+}
+
+// ZeroCrossingSolver<K,T> (zero_crossing.hpp:20-90).
+template <int K, class T>
+__device__ __forceinline__ bool solve_k(const T (*values)[K], T* lambda) {
+  T a[K][K], x[K];
+  for (int j = 0; j < K; ++j) {
+    for (int i = 0; i < K; ++i) a[j][i] = values[i][j] - values[K][j];
+    x[j] = -values[K][j];
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    int p = i;
+#pragma unroll
+    for (int j = i + 1; j < K; ++j)
+      if (fabs(a[j][i]) > fabs(a[p][i])) p = j;
+#pragma unroll
+    for (int r = i + 1; r < K; ++r) {
+      if (p == r) {
+#pragma unroll
+        for (int l = 0; l < K; ++l) {
+          const T t = a[i][l];
+          a[i][l] = a[r][l];
+          a[r][l] = t;
+        }
+        const T t = x[i];
+        x[i] = x[r];
+        x[r] = t;
+      }
+    }
+    if (fabs(a[i][i]) < T(1e-12)) return false;
+#pragma unroll
+    for (int j = i + 1; j < K; ++j) {
+      const T f = a[j][i] / a[i][i];
+      x[j] = x[j] - f * x[i];
+#pragma unroll
+      for (int l = i + 1; l < K; ++l) a[j][l] = a[j][l] - f * a[i][l];
+    }
+  }
+#pragma unroll
+  for (int i = K - 1; i >= 0; --i) {
+    T s = T(0);
+#pragma unroll
+    for (int j = i + 1; j < K; ++j) s = s + a[i][j] * x[j];
+    x[i] = (x[i] - s) / a[i][i];
+  }
+  T sum = T(0);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    lambda[i] = x[i];
+    sum = sum + lambda[i];
+  }
+  lambda[K] = T(1.0) - sum;
+  const T eps = T(1e-9);
+#pragma unroll
+  for (int i = 0; i <= K; ++i)
+    if (lambda[i] < -eps || lambda[i] > T(1.0) + eps) return false;
+  return true;
+}
+
+// ZeroCrossingSolver<1,T> (zero_crossing.hpp:93-105).
+template <class T>
+__device__ __forceinline__ bool solve_1(T v0, T v1, T* lambda) {
+  const T d = v0 - v1;
+  if (fabs(d) < T(1e-12)) return false;
+  lambda[0] = -v1 / d;
+  lambda[1] = T(1.0) - lambda[0];
+  const T eps = T(1e-9);
+  return lambda[0] >= -eps && lambda[0] <= T(1.0) + eps;
+}
+
+}  // namespace ftk2h

@@ -1,0 +1,373 @@
+// Host side of libftk2h: contexts, dispatch over (predicate, d, m, dtype), result
+// objects and the C ABI declared in include/ftk2h.h.
+#include "internal.hpp"
+
+using namespace ftk2h;
+using namespace ftk2h_internal;
+
+namespace ftk2h_internal {
+std::string& error_slot() {
+  thread_local std::string s;
+  return s;
+}
+extern template int run_pipeline<KIND_CP, 4, 3, float>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 4, 3, double>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 3, 2, float>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 3, 2, double>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 4, 2, float>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 4, 2, double>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 2, 1, float>(RunArgs&);
+extern template int run_pipeline<KIND_CP, 2, 1, double>(RunArgs&);
+extern template int run_pipeline<KIND_CONTOUR, 4, 1, float>(RunArgs&);
+extern template int run_pipeline<KIND_CONTOUR, 4, 1, double>(RunArgs&);
+extern template int run_pipeline<KIND_CONTOUR, 3, 1, float>(RunArgs&);
+extern template int run_pipeline<KIND_CONTOUR, 3, 1, double>(RunArgs&);
+extern template int run_pipeline<KIND_CONTOUR, 2, 1, float>(RunArgs&);
+extern template int run_pipeline<KIND_CONTOUR, 2, 1, double>(RunArgs&);
+
+struct Entry {
+  int kind, d, m, dtype;
+  RunFn fn;
+};
+
+template <int KIND, int D, int M>
+constexpr Entry entry_f() {
+  return {KIND, D, M, FTK2H_F32, &run_pipeline<KIND, D, M, float>};
+}
+template <int KIND, int D, int M>
+constexpr Entry entry_d() {
+  return {KIND, D, M, FTK2H_F64, &run_pipeline<KIND, D, M, double>};
+}
+
+const Entry kEntries[] = {
+    // critical points
+    entry_d<KIND_CP, 4, 3>(), entry_f<KIND_CP, 4, 3>(),
+    entry_d<KIND_CP, 3, 2>(), entry_f<KIND_CP, 3, 2>(),
+    entry_d<KIND_CP, 4, 2>(), entry_f<KIND_CP, 4, 2>(),
+    entry_d<KIND_CP, 2, 1>(), entry_f<KIND_CP, 2, 1>(),
+    // contour
+    entry_d<KIND_CONTOUR, 4, 1>(), entry_f<KIND_CONTOUR, 4, 1>(),
+    entry_d<KIND_CONTOUR, 3, 1>(), entry_f<KIND_CONTOUR, 3, 1>(),
+    entry_d<KIND_CONTOUR, 2, 1>(), entry_f<KIND_CONTOUR, 2, 1>(),
+};
+
+RunFn find_entry(int kind, int d, int m, int dtype) {
+  for (const auto& e : kEntries)
+    if (e.kind == kind && e.d == d && e.m == m && e.dtype == dtype) return e.fn;
+  return nullptr;
+}
+
+}  // namespace
+
+namespace {
+// Vertex ids of the node with key k: smallest vertex + chain offsets.
+template <int D, int M>
+void decode_keys_t(const ftk2h_result* r, const std::vector<uint64_t>& keys, uint64_t* verts) {
+  constexpr auto C = build_chains<D, M>();
+  uint64_t gs[4] = {0, 0, 0, 0}, s = 1;
+  for (int j = 0; j < D; ++j) {
+    gs[j] = s;
+    s *= r->gdims[j];
+  }
+  for (size_t i = 0; i < keys.size(); ++i) {
+    const uint64_t vmin = keys[i] / (uint64_t)C.n;
+    const int t = (int)(keys[i] % (uint64_t)C.n);
+    for (int k = 0; k <= M; ++k) {
+      uint64_t off = 0;
+      for (int j = 0; j < D; ++j)
+        if ((C.s[t][k] >> j) & 1) off += gs[j];
+      verts[i * (M + 1) + k] = vmin + off;
+    }
+  }
+}
+int decode_keys(const ftk2h_result* r, const std::vector<uint64_t>& keys, uint64_t* verts) {
+  switch (r->d * 10 + r->m) {
+    case 21: decode_keys_t<2, 1>(r, keys, verts); break;
+    case 22: decode_keys_t<2, 2>(r, keys, verts); break;
+    case 31: decode_keys_t<3, 1>(r, keys, verts); break;
+    case 32: decode_keys_t<3, 2>(r, keys, verts); break;
+    case 33: decode_keys_t<3, 3>(r, keys, verts); break;
+    case 41: decode_keys_t<4, 1>(r, keys, verts); break;
+    case 42: decode_keys_t<4, 2>(r, keys, verts); break;
+    case 43: decode_keys_t<4, 3>(r, keys, verts); break;
+    default: return fail(FTK2H_E_UNSUPPORTED, "cannot decode keys");
+  }
+  return FTK2H_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* ftk2h_last_error(void) { return error_slot().c_str(); }
+
+int ftk2h_ctx_create(int device, ftk2h_ctx** out) {
+  if (!out) return fail(FTK2H_E_INVALID, "out is NULL");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(FTK2H_E_INVALID, "no such device");
+  auto ctx = std::make_unique<ftk2h_ctx>();
+  ctx->device = device;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+  *out = ctx.release();
+  return FTK2H_OK;
+}
+
+void ftk2h_ctx_destroy(ftk2h_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->mask.release();
+  ctx->node_off.release();
+  ctx->bsum.release();
+  ctx->ebsum.release();
+  for (auto& f : ctx->fields) f.release();
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred,
+                          const ftk2h_field_view* comps, int ncomp, const ftk2h_field_view* scalar,
+                          const ftk2h_options* opt, ftk2h_result** out, void* hip_stream) {
+  if (!ctx || !mesh || !pred || !out) return fail(FTK2H_E_INVALID, "null argument");
+  *out = nullptr;
+  const int d = mesh->ndims;
+  if (d < 2 || d > 4) return fail(FTK2H_E_UNSUPPORTED, "ndims must be 2..4");
+  const int m = pred->codimension;
+  int need = 0;
+  switch (pred->kind) {
+    case FTK2H_PRED_CRITICAL_POINT:
+      need = m;
+      break;
+    case FTK2H_PRED_CONTOUR:
+      need = 1;
+      if (m != 1) return fail(FTK2H_E_INVALID, "contour codimension must be 1");
+      break;
+    case FTK2H_PRED_FIBER:
+      need = 2;
+      if (m != 2) return fail(FTK2H_E_INVALID, "fiber codimension must be 2");
+      break;
+    default:
+      return fail(FTK2H_E_INVALID, "unknown predicate kind");
+  }
+  if (pred->dtype != FTK2H_F32 && pred->dtype != FTK2H_F64) return fail(FTK2H_E_INVALID, "bad dtype");
+  if (ncomp != need || !comps) return fail(FTK2H_E_INVALID, "wrong number of field components");
+  RunFn fn = find_entry(pred->kind, d, m, pred->dtype);
+  if (!fn)
+    return fail(FTK2H_E_UNSUPPORTED, "predicate kind " + std::to_string(pred->kind) + " with d=" +
+                                         std::to_string(d) + " m=" + std::to_string(m) + " not built");
+  uint64_t nvert = 1;
+  for (int j = 0; j < d; ++j) {
+    if (mesh->local_dims[j] == 0) return fail(FTK2H_E_INVALID, "zero extent");
+    if (mesh->global_dims[j] < mesh->local_dims[j] + mesh->offset[j])
+      return fail(FTK2H_E_INVALID, "local dims + offset exceed global dims");
+    nvert *= mesh->local_dims[j];
+  }
+  for (int c = 0; c < ncomp; ++c)
+    if (!comps[c].ptr) return fail(FTK2H_E_INVALID, "null field pointer");
+
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+
+  auto res = std::make_unique<ftk2h_result>();
+  res->device = ctx->device;
+  res->kind = pred->kind;
+  res->d = d;
+  res->m = m;
+  res->dtype = pred->dtype;
+  for (int j = 0; j < d; ++j) res->gdims[j] = mesh->global_dims[j];
+
+  bool has_cells = true;
+  for (int j = 0; j < d; ++j) has_cells = has_cells && mesh->local_dims[j] >= 2;
+  if (!has_cells) {  // no hypercube: the reference visits nothing
+    *out = res.release();
+    return FTK2H_OK;
+  }
+
+  KParams P;
+  std::memset(&P, 0, sizeof(P));
+  for (int j = 0; j < 4; ++j) {
+    P.dims[j] = j < d ? (int64_t)mesh->local_dims[j] : 1;
+    P.off[j] = j < d ? (int64_t)mesh->offset[j] : 0;
+  }
+  int64_t gs = 1;
+  for (int j = 0; j < d; ++j) {
+    P.gstride[j] = gs;
+    gs *= (int64_t)mesh->global_dims[j];
+  }
+  P.nvert = (int64_t)nvert;
+  P.small = nvert < (1ull << 31);
+  P.Q = pred->sos_q;
+  P.thr[0] = pred->thresholds[0];
+  P.thr[1] = pred->thresholds[1];
+  const size_t esz = pred->dtype == FTK2H_F64 ? 8 : 4;
+
+  // Field views: device views are read in place; host views are packed into a
+  // contiguous (x fastest) device copy.
+  auto bind = [&](const ftk2h_field_view& fv, int slot, const void** ptr, int64_t* strides) -> int {
+    if (fv.on_device) {
+      *ptr = fv.ptr;
+      for (int j = 0; j < 4; ++j) strides[j] = j < d ? fv.strides[j] : 0;
+      return FTK2H_OK;
+    }
+    std::vector<char> pack(nvert * esz);
+    const char* src = static_cast<const char*>(fv.ptr);
+    int64_t lc[4] = {0, 0, 0, 0};
+    for (uint64_t v = 0; v < nvert; ++v) {
+      int64_t idx = 0;
+      for (int j = 0; j < d; ++j) idx += lc[j] * fv.strides[j];
+      std::memcpy(&pack[v * esz], src + idx * (int64_t)esz, esz);
+      for (int j = 0; j < d; ++j) {
+        if (++lc[j] < (int64_t)mesh->local_dims[j]) break;
+        lc[j] = 0;
+      }
+    }
+    int rc = ctx->fields[slot].ensure(nvert * esz);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->fields[slot].p, pack.data(), nvert * esz, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *ptr = ctx->fields[slot].p;
+    int64_t s = 1;
+    for (int j = 0; j < 4; ++j) {
+      strides[j] = j < d ? s : 0;
+      if (j < d) s *= (int64_t)mesh->local_dims[j];
+    }
+    return FTK2H_OK;
+  };
+  for (int c = 0; c < ncomp; ++c) {
+    int rc = bind(comps[c], c, &P.comp[c], P.cst[c]);
+    if (rc) return rc;
+  }
+  if (scalar && scalar->ptr && pred->kind == FTK2H_PRED_CRITICAL_POINT) {
+    int rc = bind(*scalar, 3, &P.scal, P.sst);
+    if (rc) return rc;
+  }
+
+  const int64_t nb = (int64_t)((nvert + BLOCK - 1) / BLOCK);
+  int rc;
+  if ((rc = ctx->mask.ensure(nvert * 8))) return rc;
+  if ((rc = ctx->node_off.ensure(nvert * 4))) return rc;
+  if ((rc = ctx->bsum.ensure((nb + 1) * 4))) return rc;
+  if ((rc = ctx->ebsum.ensure((nb + 1) * 4))) return rc;
+  P.mask = static_cast<uint64_t*>(ctx->mask.p);
+  P.node_off = static_cast<uint32_t*>(ctx->node_off.p);
+  P.bsum = static_cast<uint32_t*>(ctx->bsum.p);
+  P.ebsum = static_cast<uint32_t*>(ctx->ebsum.p);
+
+  RunArgs A{ctx, st, P, nb, opt ? opt->node_capacity : 0, opt ? opt->cell_capacity : 0, true, res.get()};
+  rc = fn(A);
+  if (rc) return rc;
+  *out = res.release();
+  return FTK2H_OK;
+}
+
+int ftk2h_result_sizes(const ftk2h_result* r, uint64_t* n_nodes, int* node_dim, int* n_conn) {
+  if (!r) return fail(FTK2H_E_INVALID, "null result");
+  if (n_nodes) *n_nodes = r->n_nodes;
+  if (node_dim) *node_dim = r->m;
+  if (n_conn) *n_conn = r->n_conn;
+  return FTK2H_OK;
+}
+
+int ftk2h_result_conn_info(const ftk2h_result* r, int i, int* dim, uint64_t* n_cells) {
+  if (!r || i < 0 || i >= r->n_conn) return fail(FTK2H_E_INVALID, "bad connectivity index");
+  if (dim) *dim = r->conn_dim[i];
+  if (n_cells) *n_cells = r->conn_cells[i];
+  return FTK2H_OK;
+}
+
+
+int ftk2h_result_copy_keys(const ftk2h_result* r, uint64_t* keys) {
+  if (!r || !keys) return fail(FTK2H_E_INVALID, "null argument");
+  if (!r->n_nodes) return FTK2H_OK;
+  HIP_TRY(hipSetDevice(r->device));
+  HIP_TRY(hipMemcpy(keys, r->keys, r->n_nodes * 8, hipMemcpyDeviceToHost));
+  return FTK2H_OK;
+}
+
+int ftk2h_result_copy_nodes(const ftk2h_result* r, uint64_t* vertices, double* lambda, float* bary,
+                            float* scalar, uint64_t* track_id) {
+  if (!r) return fail(FTK2H_E_INVALID, "null result");
+  const uint64_t n = r->n_nodes;
+  if (!n) return FTK2H_OK;
+  HIP_TRY(hipSetDevice(r->device));
+  const int nv = r->m + 1;
+  if (vertices) {
+    std::vector<uint64_t> keys(n);
+    HIP_TRY(hipMemcpy(keys.data(), r->keys, n * 8, hipMemcpyDeviceToHost));
+    int rc = decode_keys(r, keys, vertices);
+    if (rc) return rc;
+  }
+  if (lambda || bary) {
+    if (r->dtype == FTK2H_F64) {
+      std::vector<double> lam(n * nv);
+      HIP_TRY(hipMemcpy(lam.data(), r->lambda, n * nv * 8, hipMemcpyDeviceToHost));
+      for (uint64_t i = 0; i < n * nv; ++i) {
+        if (lambda) lambda[i] = lam[i];
+        if (bary) bary[i] = (float)lam[i];
+      }
+    } else {
+      std::vector<float> lam(n * nv);
+      HIP_TRY(hipMemcpy(lam.data(), r->lambda, n * nv * 4, hipMemcpyDeviceToHost));
+      for (uint64_t i = 0; i < n * nv; ++i) {
+        if (lambda) lambda[i] = (double)lam[i];
+        if (bary) bary[i] = lam[i];
+      }
+    }
+  }
+  if (scalar) {
+    if (r->scalar) {
+      HIP_TRY(hipMemcpy(scalar, r->scalar, n * 4, hipMemcpyDeviceToHost));
+    } else {
+      // ContourPredicate stores its threshold; other predicates store 0 (filled
+      // on device for contour through the scalar buffer when present).
+      for (uint64_t i = 0; i < n; ++i) scalar[i] = 0.f;
+    }
+  }
+  if (track_id) {
+    std::vector<uint32_t> tr(n);
+    HIP_TRY(hipMemcpy(tr.data(), r->track, n * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i) track_id[i] = tr[i];
+  }
+  return FTK2H_OK;
+}
+
+int ftk2h_result_copy_conn(const ftk2h_result* r, int i, uint32_t* indices) {
+  if (!r || i < 0 || i >= r->n_conn || !indices) return fail(FTK2H_E_INVALID, "bad argument");
+  HIP_TRY(hipSetDevice(r->device));
+  const uint64_t n = r->conn_cells[i] * (uint64_t)(r->conn_dim[i] + 1);
+  if (n) HIP_TRY(hipMemcpy(indices, r->conn[i], n * 4, hipMemcpyDeviceToHost));
+  return FTK2H_OK;
+}
+
+int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out) {
+  if (!r || !out) return fail(FTK2H_E_INVALID, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->n_nodes = r->n_nodes;
+  out->node_dim = r->m;
+  out->keys = r->keys;
+  out->lambda = r->lambda;
+  out->track_id = r->track;
+  out->scalar = r->scalar;
+  out->n_conn = r->n_conn;
+  for (int i = 0; i < 3; ++i) {
+    out->conn_dim[i] = r->conn_dim[i];
+    out->conn_cells[i] = r->conn_cells[i];
+    out->conn[i] = r->conn[i];
+  }
+  return FTK2H_OK;
+}
+
+int ftk2h_result_phase_ms(const ftk2h_result* r, float* ms, int n) {
+  if (!r || !ms) return fail(FTK2H_E_INVALID, "null argument");
+  for (int i = 0; i < n && i < NPHASE; ++i) ms[i] = r->phase_ms[i];
+  return FTK2H_OK;
+}
+
+void ftk2h_result_free(ftk2h_result* r) { delete r; }
+
+}  // extern "C"

@@ -1,0 +1,964 @@
+// Device kernels of the simplicial extraction pipeline on a regular spacetime mesh.
+//
+// One thread per mesh vertex v (local linear index, x fastest).  Vertex v owns every
+// simplex whose smallest vertex is v ("node key" = global id(v) * n_types + type),
+// and, when v is a hypercube base, the D! Kuhn cells of that hypercube.
+//
+//   classify   per vertex: bit mask over the n_types simplices owned by v that pass
+//              the predicate (exact SoS + solver gate for critical points).  The node
+//              set is every m-face of every d-cell that passes: the LUT pass of
+//              engine.hpp:118-158 plus the upper-boundary faces that the manifold
+//              pass of engine.hpp:1342-1367 adds.
+//   offsets    canonical index of each node = exclusive prefix of mask popcounts in
+//              vertex order (Simplex order, mesh.hpp:36-42), and per-vertex counts of
+//              manifold cells.
+//   emit       solver output lambda for every node; manifold cells in canonical
+//              indices: star edges / star-fan triangles (engine.hpp:1342-1417) or
+//              marching pentatope/tetrahedron tets/triangles (engine.hpp:1210-1340).
+//              Every cell of hypercube v contains a node owned by v as its smallest
+//              index (k = 1 and marching), so per-vertex segments sorted locally are
+//              globally sorted (complex.hpp:27-58); k = 2 cells are sorted globally.
+//   link       union-find by hooking the larger root under the smaller
+//              (engine.hpp:46-74): root = minimum canonical index = track_id.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "exact.hpp"
+#include "tables.hpp"
+
+namespace ftk2h {
+
+constexpr int KIND_CP = 0, KIND_CONTOUR = 1, KIND_FIBER = 2;
+constexpr int BLOCK = 256;
+
+struct KParams {
+  int64_t dims[4];     // local extents (1 beyond d)
+  int64_t off[4];      // global offset of local (0..0)
+  int64_t gstride[4];  // global id strides
+  int64_t nvert;       // local vertices
+  int small;           // nvert < 2^31: 32-bit index arithmetic
+  const void* comp[3];
+  int64_t cst[3][4];
+  const void* scal;
+  int64_t sst[4];
+  double Q;
+  double thr[2];
+  uint64_t* mask;
+  uint32_t* node_off;
+  uint32_t* bsum;   // per-block node counts -> exclusive offsets (+ total at [nb])
+  uint32_t* ebsum;  // per-block cell counts  -> exclusive offsets (+ total at [nb])
+  uint64_t* keys;
+  void* lambda;
+  float* scalar_out;
+  uint32_t* cells;
+  uint32_t* parent;
+  uint32_t* track;
+  uint64_t n_nodes;
+  uint64_t n_cells;
+};
+
+// ---------------------------------------------------------------------------
+// compile-time loop
+template <int I, int N, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor<I + 1, N>(f);
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void decompose(const KParams& P, int64_t v, int64_t* l) {
+  if (P.small) {
+    uint32_t r = (uint32_t)v;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const uint32_t dj = (uint32_t)P.dims[j];
+      l[j] = r % dj;
+      r /= dj;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      l[j] = v % P.dims[j];
+      v /= P.dims[j];
+    }
+  }
+}
+
+// local linear index of l + s
+template <int D>
+__device__ __forceinline__ int64_t lin_of(const KParams& P, const int64_t* l, int s) {
+  int64_t r = 0, mul = 1;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    r += (l[j] + ((s >> j) & 1)) * mul;
+    mul *= P.dims[j];
+  }
+  return r;
+}
+
+template <int D>
+__device__ __forceinline__ uint64_t gid_of(const KParams& P, const int64_t* l, int s) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) r += (uint64_t)(l[j] + ((s >> j) & 1) + P.off[j]) * (uint64_t)P.gstride[j];
+  return r;
+}
+
+// bit s set <=> corner l + s lies inside the local grid
+template <int D>
+__device__ __forceinline__ uint32_t corner_mask(const KParams& P, const int64_t* l) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int s = 0; s < (1 << D); ++s) {
+    bool in = true;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if ((s >> j) & 1) in = in && (l[j] + 1 < P.dims[j]);
+    if (in) m |= 1u << s;
+  }
+  return m;
+}
+
+template <class T, int D>
+__device__ __forceinline__ T load_val(const KParams& P, int c, const int64_t* l, int s) {
+  const T* p = static_cast<const T*>(P.comp[c]);
+  int64_t idx = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) idx += (l[j] + ((s >> j) & 1)) * P.cst[c][j];
+  return p[idx];
+}
+
+template <class T, int D>
+__device__ __forceinline__ T load_scalar(const KParams& P, const int64_t* l, int s) {
+  const T* p = static_cast<const T*>(P.scal);
+  int64_t idx = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) idx += (l[j] + ((s >> j) & 1)) * P.sst[j];
+  return p[idx];
+}
+
+// chains packed 4 bits per vertex offset, for run-time type lookups
+template <int D, int M>
+struct PackedChains {
+  uint32_t v[64];
+};
+template <int D, int M>
+constexpr PackedChains<D, M> build_packed() {
+  PackedChains<D, M> p{};
+  constexpr auto C = build_chains<D, M>();
+  for (int t = 0; t < C.n; ++t) {
+    uint32_t w = 0;
+    for (int i = 1; i <= M; ++i) w |= (uint32_t)C.s[t][i] << (4 * (i - 1));
+    p.v[t] = w;
+  }
+  return p;
+}
+template <int D, int M>
+__device__ __forceinline__ uint32_t packed_chain(int t) {
+  constexpr PackedChains<D, M> P = build_packed<D, M>();
+  return P.v[t];
+}
+
+// Table references usable inside lambdas without capture.
+template <int D, int M>
+constexpr const ChainTable<D, M>& CH = Chains<D, M>::table;
+template <int D, int M>
+constexpr const CellFaceTable<D, M>& CF = CellFaces<D, M>::table;
+
+// index of triangle (0,a,b) among Chains<D,2>
+template <int D>
+constexpr int tri_index(int a, int b) {
+  constexpr const auto& C = Chains<D, 2>::table;
+  for (int t = 0; t < C.n; ++t)
+    if (C.s[t][1] == a && C.s[t][2] == b) return t;
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Triple product sign with an error-bounded FP64 filter and an exact fallback.
+// Inputs are quantized integers |q| < 2^20 held exactly in float.
+__device__ __forceinline__ void cross3(const float* A, const float* B, double* e) {
+  // exact: products < 2^40, differences < 2^41
+  e[0] = fma((double)A[1], (double)B[2], -((double)A[2] * (double)B[1]));
+  e[1] = fma((double)A[2], (double)B[0], -((double)A[0] * (double)B[2]));
+  e[2] = fma((double)A[0], (double)B[1], -((double)A[1] * (double)B[0]));
+}
+
+// sign of e . C where e = A x B (so the value is det[A; B; C])
+__device__ __forceinline__ int dot_sign(const double* e, const float* C, double bnd) {
+  double f = e[0] * (double)C[0];
+  f = fma(e[1], (double)C[1], f);
+  f = fma(e[2], (double)C[2], f);
+  if (f > bnd) return 1;
+  if (f < -bnd) return -1;
+  // |value| <= 3 * 2^41 * 2^20 < 2^63: exact in int64
+  const int64_t s = (int64_t)e[0] * (int64_t)C[0] + (int64_t)e[1] * (int64_t)C[1] +
+                    (int64_t)e[2] * (int64_t)C[2];
+  return s > 0 ? 1 : (s < 0 ? -1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// Critical points of a 3-component field on 3-simplices (CriticalPointPredicate<3,T>,
+// predicate.hpp:66-99).
+
+template <class T, int D>
+__device__ __noinline__ bool cp3_full(const KParams& P, const int64_t* l, int a, int b, int c) {
+  const int off[4] = {0, a, b, c};
+  T vals[4][3];
+  int64_t X[4][3];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) {
+      vals[i][k] = load_val<T, D>(P, k, l, off[i]);
+      X[i][k] = q64(vals[i][k], P.Q);
+    }
+  if (!inside3_exact(X)) return false;
+  T lam[4];
+  return solve_k<3, T>(vals, lam);
+}
+
+template <class T, int D>
+__device__ __forceinline__ bool cp3_solve_ok(const KParams& P, const int64_t* l, int a, int b, int c) {
+  const int off[4] = {0, a, b, c};
+  T vals[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vals[i][k] = load_val<T, D>(P, k, l, off[i]);
+  T lam[4];
+  return solve_k<3, T>(vals, lam);
+}
+
+template <class T, int D>
+__device__ __forceinline__ uint64_t classify_cp3(const KParams& P, const int64_t* l) {
+  constexpr int NC = 1 << D;
+  const uint32_t inb = corner_mask<D>(P, l);
+  float X[NC][3];
+  int64_t mx = 0;
+#pragma unroll
+  for (int s = 0; s < NC; ++s) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      int64_t q = 0;
+      if ((inb >> s) & 1) q = q64(load_val<T, D>(P, k, l, s), P.Q);
+      const int64_t aq = q < 0 ? -q : q;
+      mx = aq > mx ? aq : mx;
+      X[s][k] = (float)q;
+    }
+  }
+  uint64_t mask = 0;
+  if (mx >= (int64_t(1) << 20)) {
+    // magnitudes beyond the fast filter: verbatim exact path for every tet
+    sfor<0, CH<D, 3>.n>([&](auto tt) {
+      constexpr int t = decltype(tt)::value;
+      constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
+      if ((inb >> c) & 1)
+        if (cp3_full<T, D>(P, l, a, b, c)) mask |= 1ull << t;
+    });
+    return mask;
+  }
+  const double md = (double)mx;
+  const double bnd = md * md * md * 0x1p-48;
+  uint64_t tpos = 0, tzero = 0;
+  sfor<0, CH<D, 2>.n>([&](auto tt) {
+    constexpr int t = decltype(tt)::value;
+    constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
+    if ((inb >> b) & 1) {
+      double e[3];
+      cross3(X[0], X[a], e);
+      const int sg = dot_sign(e, X[b], bnd);
+      if (sg > 0) tpos |= 1ull << t;
+      if (sg == 0) tzero |= 1ull << t;
+    }
+  });
+  sfor<0, CH<D, 3>.n>([&](auto tt) {
+    constexpr int t = decltype(tt)::value;
+    constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
+    constexpr int i012 = tri_index<D>(a, b), i013 = tri_index<D>(a, c), i023 = tri_index<D>(b, c);
+    if ((inb >> c) & 1) {
+      const bool z = ((tzero >> i012) | (tzero >> i013) | (tzero >> i023)) & 1;
+      // d3 = det3(v0,v1,v2) = tp(0ab); d2 = det3(v0,v3,v1) = -tp(0ac);
+      // d1 = det3(v0,v2,v3) = tp(0bc); d0 = det3(v1,v3,v2) = -tp(abc)
+      const int p012 = (tpos >> i012) & 1, p013 = (tpos >> i013) & 1, p023 = (tpos >> i023) & 1;
+      bool acc = false;
+      if (z) {
+        acc = cp3_full<T, D>(P, l, a, b, c);
+      } else if (p012 != p013 && p023 == p012) {
+        double e[3];
+        cross3(X[b], X[c], e);
+        const int sabc = dot_sign(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
+        if (sabc == 0) {
+          acc = cp3_full<T, D>(P, l, a, b, c);
+        } else {
+          const int d0 = -sabc, d3 = p012 ? 1 : -1;
+          if (d0 == d3) acc = cp3_solve_ok<T, D>(P, l, a, b, c);
+        }
+      }
+      if (acc) mask |= 1ull << t;
+    }
+  });
+  return mask;
+}
+
+// ---------------------------------------------------------------------------
+// det2-based predicates on 2-simplices: CriticalPointPredicate<2,T> and
+// FiberPredicate<T> (values minus thresholds, predicate.hpp:352).
+
+template <int KIND, class T, int D>
+__device__ __forceinline__ T val2(const KParams& P, int k, const int64_t* l, int s) {
+  const T v = load_val<T, D>(P, k, l, s);
+  if constexpr (KIND == KIND_FIBER) return v - (T)P.thr[k];
+  return v;
+}
+
+template <int KIND, class T, int D>
+__device__ __noinline__ bool tri_full(const KParams& P, const int64_t* l, int a, int b) {
+  const int off[3] = {0, a, b};
+  T vals[3][2];
+  int64_t X[3][2];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 2; ++k) {
+      vals[i][k] = val2<KIND, T, D>(P, k, l, off[i]);
+      X[i][k] = q64(vals[i][k], P.Q);
+    }
+  if (!inside2_exact(X)) return false;
+  if constexpr (KIND == KIND_FIBER) return true;  // solver failure falls back to 1/3
+  T lam[3];
+  return solve_k<2, T>(vals, lam);
+}
+
+__device__ __forceinline__ double cross2(const double* A, const double* B) {
+  // exact for |q| < 2^26: products < 2^52
+  return fma(A[0], B[1], -(A[1] * B[0]));
+}
+
+template <int KIND, class T, int D>
+__device__ __forceinline__ uint64_t classify_tri(const KParams& P, const int64_t* l) {
+  constexpr int NC = 1 << D;
+  const uint32_t inb = corner_mask<D>(P, l);
+  double X[NC][2];
+  int64_t mx = 0;
+#pragma unroll
+  for (int s = 0; s < NC; ++s) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      int64_t q = 0;
+      if ((inb >> s) & 1) q = q64(val2<KIND, T, D>(P, k, l, s), P.Q);
+      const int64_t aq = q < 0 ? -q : q;
+      mx = aq > mx ? aq : mx;
+      X[s][k] = (double)q;
+    }
+  }
+  const bool slow = mx >= (int64_t(1) << 26);
+  uint64_t mask = 0;
+  sfor<0, CH<D, 2>.n>([&](auto tt) {
+    constexpr int t = decltype(tt)::value;
+    constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
+    if ((inb >> b) & 1) {
+      bool acc;
+      if (slow) {
+        acc = tri_full<KIND, T, D>(P, l, a, b);
+      } else {
+        const double c01 = cross2(X[0], X[a]);
+        const double c12 = cross2(X[a], X[b]);
+        const double c20 = cross2(X[b], X[0]);
+        const int s0 = c01 < 0 ? -1 : 1;  // tie: i0 < i1 -> +1
+        const int s1 = c12 < 0 ? -1 : 1;  // tie: i1 < i2 -> +1
+        const int s2 = c20 > 0 ? 1 : -1;  // tie: i2 > i0 -> -1
+        acc = (s0 == s1) && (s1 == s2);
+        if constexpr (KIND == KIND_CP) {
+          if (acc) {
+            T vals[3][2];
+            const int off[3] = {0, a, b};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int k = 0; k < 2; ++k) vals[i][k] = load_val<T, D>(P, k, l, off[i]);
+            T lam[3];
+            acc = solve_k<2, T>(vals, lam);
+          }
+        }
+      }
+      if (acc) mask |= 1ull << t;
+    }
+  });
+  return mask;
+}
+
+// ---------------------------------------------------------------------------
+// Edge predicates: ContourPredicate<T> (sign change of value - threshold) and
+// CriticalPointPredicate<1,T> (sign change + solver gate).
+template <int KIND, class T, int D>
+__device__ __forceinline__ uint64_t classify_edge(const KParams& P, const int64_t* l) {
+  constexpr int NC = 1 << D;
+  const uint32_t inb = corner_mask<D>(P, l);
+  int sg[NC];
+  T v[NC];
+#pragma unroll
+  for (int s = 0; s < NC; ++s) {
+    v[s] = T(0);
+    sg[s] = 1;
+    if ((inb >> s) & 1) {
+      v[s] = load_val<T, D>(P, 0, l, s);
+      if constexpr (KIND == KIND_CONTOUR) v[s] = v[s] - (T)P.thr[0];
+      sg[s] = sos_sign(v[s], gid_of<D>(P, l, s), P.Q);
+    }
+  }
+  uint64_t mask = 0;
+#pragma unroll
+  for (int s = 1; s < NC; ++s) {
+    if (((inb >> s) & 1) && sg[s] != sg[0]) {
+      bool acc = true;
+      if constexpr (KIND == KIND_CP) {
+        T lam[2];
+        acc = solve_1<T>(v[0], v[s], lam);
+      }
+      if (acc) mask |= 1ull << (s - 1);  // Chains<D,1> type of (0, s) is s - 1
+    }
+  }
+  return mask;
+}
+
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ uint64_t classify_vertex(const KParams& P, const int64_t* l) {
+  if constexpr (M == 3) {
+    return classify_cp3<T, D>(P, l);
+  } else if constexpr (M == 2) {
+    return classify_tri<KIND, T, D>(P, l);
+  } else {
+    return classify_edge<KIND, T, D>(P, l);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// lambda of one node (type t at vertex l), written as T[M+1]; returns scalar.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ void node_lambda(const KParams& P, const int64_t* l, int t, T* lam,
+                                            float* scalar) {
+  const uint32_t pc = packed_chain<D, M>(t);
+  int off[M + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
+  *scalar = 0.f;
+  if constexpr (M == 1) {
+    T v0 = load_val<T, D>(P, 0, l, off[0]);
+    T v1 = load_val<T, D>(P, 0, l, off[1]);
+    if constexpr (KIND == KIND_CONTOUR) {
+      v0 = v0 - (T)P.thr[0];
+      v1 = v1 - (T)P.thr[0];
+      if (!solve_1<T>(v0, v1, lam)) lam[0] = lam[1] = (T)0.5;
+      *scalar = (float)(T)P.thr[0];
+    } else {
+      solve_1<T>(v0, v1, lam);
+    }
+  } else {
+    T vals[M + 1][M];
+#pragma unroll
+    for (int i = 0; i <= M; ++i)
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        vals[i][k] = load_val<T, D>(P, k, l, off[i]);
+        if constexpr (KIND == KIND_FIBER) vals[i][k] = vals[i][k] - (T)P.thr[k];
+      }
+    const bool ok = solve_k<M, T>(vals, lam);
+    if constexpr (KIND == KIND_FIBER) {
+      if (!ok)
+#pragma unroll
+        for (int i = 0; i <= M; ++i) lam[i] = (T)(1.0 / 3.0);
+    }
+  }
+  if constexpr (KIND == KIND_CP) {
+    if (P.scal) {  // predicate.hpp:81-92
+      T sv = T(0);
+#pragma unroll
+      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * load_scalar<T, D>(P, l, off[i]);
+      *scalar = (float)sv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block-wide exclusive scan of one uint32 per thread (BLOCK threads, wave64)
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total) {
+  __shared__ uint32_t wsum[BLOCK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / 64; ++i) {
+    const uint32_t s = wsum[i];
+    if (i < w) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + incl - x;
+}
+
+// ---------------------------------------------------------------------------
+// Manifold cells of the hypercube based at l (all l_j <= dims_j - 2).
+//   Mode 0: count only.  Mode 1: write cells (canonical indices, each cell sorted).
+template <int D, int M>
+struct OwnerSet {
+  static constexpr uint32_t bits() {
+    uint32_t b = 0;
+    for (int p = 0; p < CellFaceTable<D, M>::NP; ++p)
+      for (int f = 0; f < CellFaceTable<D, M>::NF; ++f) b |= 1u << CF<D, M>.owner[p][f];
+    return b;
+  }
+};
+
+template <int KIND, int D, int M>
+struct CellKind {
+  // 1: star edges (k = 1); 2: star-fan triangles (k = 2); 3: marching pentatope
+  // (contour, d = 4); 4: marching tetrahedron (contour, d = 3); 0: none.
+  static constexpr int value = (M == 1 && (D == 4 || D == 3))
+                                   ? (KIND == KIND_CONTOUR ? (D == 4 ? 3 : 4) : 0)
+                                   : ((D - M) == 1 ? 1 : ((D - M) == 2 ? 2 : 0));
+  static constexpr int verts = value == 1 ? 2 : (value == 2 ? 3 : (value == 3 ? 4 : (value == 4 ? 3 : 0)));
+  static constexpr int dim = verts - 1;
+};
+
+template <int N>
+__device__ __forceinline__ void sort_small(uint32_t* c) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N; ++j)
+      if (c[j] < c[i]) {
+        const uint32_t t = c[i];
+        c[i] = c[j];
+        c[j] = t;
+      }
+}
+
+template <int KIND, int D, int M>
+__device__ __forceinline__ uint32_t cells_of(const KParams& P, const int64_t* l, int64_t v, int mode,
+                                             uint64_t base) {
+  constexpr int CK = CellKind<KIND, D, M>::value;
+  if constexpr (CK == 0) {
+    return 0;
+  } else {
+    constexpr int NV = CellKind<KIND, D, M>::verts;
+    constexpr int NP = CellFaceTable<D, M>::NP;
+    constexpr int NF = CellFaceTable<D, M>::NF;
+    constexpr uint32_t OWN = OwnerSet<D, M>::bits();
+    uint64_t mk[1 << D];
+    uint32_t no[1 << D];
+#pragma unroll
+    for (int o = 0; o < (1 << D); ++o) {
+      mk[o] = 0;
+      no[o] = 0;
+      if ((OWN >> o) & 1) {
+        const int64_t u = v + (o == 0 ? 0 : lin_of<D>(P, l, o) - v);
+        mk[o] = P.mask[u];
+        if (mode) no[o] = P.node_off[u];
+      }
+    }
+    auto idx = [&](int o, int t) -> uint32_t {
+      return no[o] + (uint32_t)__popcll(mk[o] & ((1ull << t) - 1ull));
+    };
+    uint32_t count = 0;
+    auto put = [&](uint32_t* c) {
+      if (mode) {
+        sort_small<NV>(c);
+        uint32_t* dst = P.cells + (base + count) * NV;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) dst[i] = c[i];
+      }
+      count++;
+    };
+    if constexpr (CK == 1 || CK == 2) {
+      sfor<0, NP>([&](auto pp) {
+        constexpr int p = decltype(pp)::value;
+        // active faces in encode_simplex_id order
+        uint32_t act = 0;  // bit k: k-th face in order is active
+        sfor<0, NF>([&](auto kk) {
+          constexpr int k = decltype(kk)::value;
+          constexpr int f = CF<D, M>.order[p][k];
+          constexpr int o = CF<D, M>.owner[p][f], t = CF<D, M>.type[p][f];
+          if ((mk[o] >> t) & 1) act |= 1u << k;
+        });
+        if (act == 0) return;
+        const int k0 = __builtin_ctz(act);
+        if constexpr (CK == 1) {
+          if (mode == 0) {
+            count += __builtin_popcount(act) - 1;
+          } else {
+            uint32_t hs = 0;
+            sfor<0, NF>([&](auto kk) {
+              constexpr int k = decltype(kk)::value;
+              constexpr int f = CF<D, M>.order[p][k];
+              if (k == k0) hs = idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f]);
+            });
+            sfor<0, NF>([&](auto kk) {
+              constexpr int k = decltype(kk)::value;
+              constexpr int f = CF<D, M>.order[p][k];
+              if (((act >> k) & 1) && k != k0) {
+                uint32_t c[2] = {hs, idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f])};
+                put(c);
+              }
+            });
+          }
+        } else {
+          // star-fan (engine.hpp:1388-1415): for each cell facet (drop vertex i)
+          uint32_t hs = 0;
+          if (mode) {
+            sfor<0, NF>([&](auto kk) {
+              constexpr int k = decltype(kk)::value;
+              constexpr int f = CF<D, M>.order[p][k];
+              if (k == k0) hs = idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f]);
+            });
+          }
+          sfor<0, D + 1>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            // faces (in order) that avoid cell vertex i
+            uint32_t onf = 0;
+            sfor<0, NF>([&](auto kk) {
+              constexpr int k = decltype(kk)::value;
+              constexpr int f = CF<D, M>.order[p][k];
+              if (!((CF<D, M>.vmask[f] >> i) & 1)) onf |= 1u << k;
+            });
+            const uint32_t sel = act & onf;
+            if (__builtin_popcount(sel) < 2) return;
+            const int kt = __builtin_ctz(sel);
+            // h_S != h_T: the star centre must not be the facet's first node
+            if (kt == k0) return;
+            // every other node n of the facet with n != h_S
+            const uint32_t rest = sel & ~(1u << kt) & ~(1u << k0);
+            if (mode == 0) {
+              count += __builtin_popcount(rest);
+            } else {
+              uint32_t ht = 0;
+              sfor<0, NF>([&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                constexpr int f = CF<D, M>.order[p][k];
+                if (k == kt) ht = idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f]);
+              });
+              sfor<0, NF>([&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                constexpr int f = CF<D, M>.order[p][k];
+                if ((rest >> k) & 1) {
+                  uint32_t c[3] = {hs, ht, idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f])};
+                  put(c);
+                }
+              });
+            }
+          });
+        }
+      });
+    } else {
+      // marching pentatope / tetrahedron on vertex signs relative to vertex 0 of
+      // the cell: edge (0, s) of hypercube v is active iff the signs differ.
+      constexpr int NPC = CellFaceTable<D, 1>::NP;
+      sfor<0, NPC>([&](auto pp) {
+        constexpr int p = decltype(pp)::value;
+        int rel[D + 1];
+        rel[0] = 0;
+#pragma unroll
+        for (int i = 1; i <= D; ++i) rel[i] = (int)((mk[0] >> (CF<D, 1>.cell[p][i] - 1)) & 1);
+        int g1 = 0;
+#pragma unroll
+        for (int i = 0; i <= D; ++i) g1 += rel[i];
+        const int g0 = D + 1 - g1;
+        // node of cell edge (i, j), i < j: owner c_i, type c_j ^ c_i - 1
+        auto eidx = [&](int i, int j) -> uint32_t {
+          int ci = 0, cj = 0;
+#pragma unroll
+          for (int q = 0; q <= D; ++q) {
+            if (q == i) ci = CF<D, 1>.cell[p][q];
+            if (q == j) cj = CF<D, 1>.cell[p][q];
+          }
+          const int lo = i < j ? ci : cj, hi = i < j ? cj : ci;
+          uint32_t r = 0;
+#pragma unroll
+          for (int o = 0; o < (1 << D); ++o)
+            if (o == lo) r = idx(o, (lo ^ hi) - 1);
+          return r;
+        };
+        // groups in position order; grp[0] holds vertex 0's side.  Indices are
+        // written through compile-time selects so the arrays stay in registers.
+        int grp[2][D + 1];
+        int gn0 = 0, gn1 = 0;
+#pragma unroll
+        for (int i = 0; i <= D; ++i) {
+#pragma unroll
+          for (int q = 0; q <= D; ++q) {
+            if (rel[i] == 0 && q == gn0) grp[0][q] = i;
+            if (rel[i] == 1 && q == gn1) grp[1][q] = i;
+          }
+          if (rel[i] == 0)
+            gn0++;
+          else
+            gn1++;
+        }
+        if constexpr (CK == 3) {
+          if (g0 == 1 || g1 == 1) {
+            const int sg = (g0 == 1) ? 0 : 1;
+            if (mode == 0) {
+              count += 1;
+            } else {
+              uint32_t c[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                c[q] = sg == 0 ? eidx(grp[0][0], grp[1][q]) : eidx(grp[1][0], grp[0][q]);
+              put(c);
+            }
+          } else if (g0 == 2 || g1 == 2) {
+            if (mode == 0) {
+              count += 3;
+            } else {
+              const int tw = (g0 == 2) ? 0 : 1;
+              uint32_t T0[3], T1[3];
+#pragma unroll
+              for (int q = 0; q < 3; ++q) {
+                T0[q] = tw == 0 ? eidx(grp[0][0], grp[1][q]) : eidx(grp[1][0], grp[0][q]);
+                T1[q] = tw == 0 ? eidx(grp[0][1], grp[1][q]) : eidx(grp[1][1], grp[0][q]);
+              }
+              uint32_t c0[4] = {T0[0], T0[1], T0[2], T1[2]};
+              uint32_t c1[4] = {T0[0], T0[1], T1[1], T1[2]};
+              uint32_t c2[4] = {T0[0], T1[0], T1[1], T1[2]};
+              put(c0);
+              put(c1);
+              put(c2);
+            }
+          }
+        } else {  // CK == 4: marching tetrahedron (d = 3)
+          if (g0 == 1 || g1 == 1) {
+            if (mode == 0) {
+              count += 1;
+            } else {
+              const int sg = (g0 == 1) ? 0 : 1;
+              uint32_t c[3];
+#pragma unroll
+              for (int q = 0; q < 3; ++q)
+                c[q] = sg == 0 ? eidx(grp[0][0], grp[1][q]) : eidx(grp[1][0], grp[0][q]);
+              put(c);
+            }
+          } else if (g0 == 2 && g1 == 2) {
+            if (mode == 0) {
+              count += 2;
+            } else {
+              uint32_t n[4];
+              int k = 0;
+#pragma unroll
+              for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) n[k++] = eidx(grp[0][a], grp[1][b]);
+              sort_small<4>(n);
+              uint32_t c0[3] = {n[0], n[1], n[2]};
+              uint32_t c1[3] = {n[0], n[2], n[3]};
+              put(c0);
+              put(c1);
+            }
+          }
+        }
+      });
+    }
+    return count;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK) void k_classify(KParams P) {
+  const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t m = 0;
+  if (v < P.nvert) {
+    int64_t l[4];
+    decompose<D>(P, v, l);
+    m = classify_vertex<KIND, D, M, T>(P, l);
+    P.mask[v] = m;
+  }
+  uint32_t tot;
+  block_excl_scan((uint32_t)__popcll(m), &tot);
+  if (threadIdx.x == 0) P.bsum[blockIdx.x] = tot;
+}
+
+// exclusive scan of n uint32 in place (single block of 1024 threads); total at a[n]
+static __global__ __launch_bounds__(1024) void k_scan_small(uint32_t* a, int64_t n) {
+  __shared__ uint32_t part[1024];
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t b = threadIdx.x * per;
+  const int64_t e = b + per < n ? b + per : n;
+  uint32_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += a[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += y;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - s;
+  for (int64_t i = b; i < e; ++i) {
+    const uint32_t x = a[i];
+    a[i] = run;
+    run += x;
+  }
+  if (threadIdx.x == 1023) a[n] = part[1023];
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
+  const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t m = 0;
+  int64_t l[4] = {0, 0, 0, 0};
+  if (v < P.nvert) {
+    m = P.mask[v];
+    decompose<D>(P, v, l);
+  }
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
+  if (v < P.nvert) P.node_off[v] = P.bsum[blockIdx.x] + ex;
+  uint32_t nc = 0;
+  if (v < P.nvert) {
+    bool hc = true;
+#pragma unroll
+    for (int j = 0; j < D; ++j) hc = hc && (l[j] + 1 < P.dims[j]);
+    if (hc) nc = cells_of<KIND, D, M>(P, l, v, 0, 0);
+  }
+  block_excl_scan(nc, &tot);
+  if (threadIdx.x == 0) P.ebsum[blockIdx.x] = tot;
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
+  constexpr int NTY = Chains<D, M>::N;
+  const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  int64_t l[4] = {0, 0, 0, 0};
+  uint64_t m = 0;
+  bool hc = false;
+  if (v < P.nvert) {
+    decompose<D>(P, v, l);
+    m = P.mask[v];
+    hc = true;
+#pragma unroll
+    for (int j = 0; j < D; ++j) hc = hc && (l[j] + 1 < P.dims[j]);
+  }
+  // nodes
+  if (m) {
+    uint32_t pos = P.node_off[v];
+    const uint64_t g = gid_of<D>(P, l, 0);
+    T* lam_out = static_cast<T*>(P.lambda);
+    uint64_t mm = m;
+    while (mm) {
+      const int t = __builtin_ctzll(mm);
+      mm &= mm - 1;
+      T lam[M + 1];
+      float sc;
+      node_lambda<KIND, D, M, T>(P, l, t, lam, &sc);
+      P.keys[pos] = g * (uint64_t)NTY + (uint64_t)t;
+#pragma unroll
+      for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
+      if (P.scalar_out) P.scalar_out[pos] = sc;
+      P.parent[pos] = pos;
+      pos++;
+    }
+  }
+  // cells
+  uint32_t nc = 0;
+  if (hc) nc = cells_of<KIND, D, M>(P, l, v, 0, 0);
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan(nc, &tot);
+  if (nc) {
+    const uint64_t base = (uint64_t)P.ebsum[blockIdx.x] + ex;
+    cells_of<KIND, D, M>(P, l, v, 1, base);
+    // sort this vertex's segment lexicographically (insertion sort, small)
+    constexpr int NV = CellKind<KIND, D, M>::verts;
+    uint32_t* seg = P.cells + base * NV;
+    for (uint32_t i = 1; i < nc; ++i) {
+      uint32_t cur[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) cur[q] = seg[i * NV + q];
+      int j = (int)i - 1;
+      while (j >= 0) {
+        bool gt = false, eq = true;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const uint32_t a = seg[j * NV + q];
+          if (eq && a != cur[q]) {
+            gt = a > cur[q];
+            eq = false;
+          }
+        }
+        if (!gt) break;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) seg[(j + 1) * NV + q] = seg[j * NV + q];
+        j--;
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) seg[(j + 1) * NV + q] = cur[q];
+    }
+  }
+}
+
+// union-find hooking (larger root under smaller) over every cell
+__device__ __forceinline__ uint32_t uf_load(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
+  uint32_t p = uf_load(parent + x);
+  while (p != x) {
+    const uint32_t gp = uf_load(parent + p);
+    if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+__device__ __forceinline__ void uf_unite(uint32_t* parent, uint32_t a, uint32_t b) {
+  while (true) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    if (a > b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t expected = b;
+    if (__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+    b = expected;
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(BLOCK) void k_link(uint32_t* parent, const uint32_t* cells, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* c = cells + i * NV;
+  const uint32_t a = c[0];
+#pragma unroll
+  for (int q = 1; q < NV; ++q) uf_unite(parent, a, c[q]);
+}
+
+static __global__ __launch_bounds__(BLOCK) void k_compress(uint32_t* parent, uint32_t* track, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  uint32_t x = (uint32_t)i;
+  uint32_t p = parent[x];
+  while (p != x) {
+    x = p;
+    p = parent[x];
+  }
+  track[i] = x;
+}
+
+}  // namespace ftk2h

@@ -1,0 +1,282 @@
+"""Python mirror of FTK2's SimplicialEngine interface over the C ABI (libftk2h.so).
+
+Mirrors, on a RegularSimplicialMesh:
+  RegularSimplicialMesh(dims, offset, global_dims)   include/ftk2/core/mesh.hpp:256-265
+  CriticalPointPredicate / ContourPredicate / FiberPredicate   predicate.hpp:49,135,340
+  SimplicialEngine(mesh, pred).execute(data, var_names) / get_complex()
+                                                     engine.hpp:84,96,822
+  FeatureComplex (vertices sorted by simplex, connectivity sorted)  complex.hpp:14-58
+
+Arrays are indexed like ftk::ndarray::f(x, y, (z), t): numpy arrays are read from
+host memory (copied to the device), torch CUDA tensors are read in place.
+Every call goes to the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+_contexts = {}
+
+
+def context(device: int = 0):
+    """Per-device ftk2h_ctx, created on first use."""
+    ctx = _contexts.get(device)
+    if ctx is None:
+        p = C.c_void_p()
+        _lib.check(_lib.lib().ftk2h_ctx_create(device, C.byref(p)))
+        ctx = p
+        _contexts[device] = ctx
+    return ctx
+
+
+class RegularSimplicialMesh:
+    def __init__(self, dims, offset=None, global_dims=None):
+        self.dims = [int(x) for x in dims]
+        self.offset = [int(x) for x in offset] if offset else [0] * len(self.dims)
+        self.global_dims = [int(x) for x in global_dims] if global_dims else list(self.dims)
+
+    def get_total_dimension(self):
+        return len(self.dims)
+
+    def desc(self):
+        d = _lib.MeshDesc()
+        d.ndims = len(self.dims)
+        for j in range(4):
+            d.local_dims[j] = self.dims[j] if j < len(self.dims) else 1
+            d.offset[j] = self.offset[j] if j < len(self.dims) else 0
+            d.global_dims[j] = self.global_dims[j] if j < len(self.dims) else 1
+        return d
+
+
+@dataclass
+class CriticalPointPredicate:
+    M: int = 3
+    var_names: list = field(default_factory=list)
+    vector_var_name: str = ""
+    scalar_var_name: str = ""
+    use_multicomponent: bool = True
+    sos_q: float = 1000000.0
+    kind = _lib.PRED_CRITICAL_POINT
+
+    @property
+    def codimension(self):
+        return self.M
+
+
+@dataclass
+class ContourPredicate:
+    var_name: str = ""
+    threshold: float = 0.0
+    sos_q: float = 1000000.0
+    kind = _lib.PRED_CONTOUR
+    codimension = 1
+
+
+@dataclass
+class FiberPredicate:
+    var_names: list = field(default_factory=lambda: ["", ""])
+    thresholds: list = field(default_factory=lambda: [0.0, 0.0])
+    sos_q: float = 1000000.0
+    kind = _lib.PRED_FIBER
+    codimension = 2
+
+
+def _is_torch_cuda(a):
+    try:
+        import torch
+        return isinstance(a, torch.Tensor) and a.is_cuda
+    except ImportError:
+        return False
+
+
+def _dtype_of(a):
+    if _is_torch_cuda(a):
+        import torch
+        if a.dtype == torch.float32:
+            return _lib.F32
+        if a.dtype == torch.float64:
+            return _lib.F64
+        raise TypeError(f"unsupported dtype {a.dtype}")
+    if a.dtype == np.float32:
+        return _lib.F32
+    if a.dtype == np.float64:
+        return _lib.F64
+    raise TypeError(f"unsupported dtype {a.dtype}")
+
+
+def field_view(a, shift=None):
+    """ftk2h_field_view of an array indexed [x, y, (z), t] by local coordinates.
+    shift: per-axis coordinate offset added to every index (multicomponent mode and
+    scalar interpolation read global coordinates, engine.hpp:1108-1121)."""
+    fv = _lib.FieldView()
+    if _is_torch_cuda(a):
+        st = list(a.stride())
+        item = a.element_size()
+        base = a.data_ptr()
+        fv.on_device = 1
+    else:
+        st = [s // a.itemsize for s in a.strides]
+        item = a.itemsize
+        base = a.ctypes.data
+        fv.on_device = 0
+    if shift is not None:
+        base += int(sum(int(o) * s for o, s in zip(shift, st))) * item
+    fv.ptr = base
+    for j in range(4):
+        fv.strides[j] = st[j] if j < len(st) else 0
+    return fv
+
+
+class Result:
+    """Owns an ftk2h_result handle."""
+
+    def __init__(self, handle, keep=None):
+        self.h = handle
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.lib().ftk2h_result_free(self.h)
+            self.h = None
+
+    def sizes(self):
+        n = C.c_uint64()
+        m = C.c_int()
+        nc = C.c_int()
+        _lib.check(_lib.lib().ftk2h_result_sizes(self.h, C.byref(n), C.byref(m), C.byref(nc)))
+        return n.value, m.value, nc.value
+
+    def phase_ms(self):
+        buf = (C.c_float * 7)()
+        _lib.check(_lib.lib().ftk2h_result_phase_ms(self.h, buf, 7))
+        return list(buf)
+
+    def device_view(self):
+        dv = _lib.DeviceView()
+        _lib.check(_lib.lib().ftk2h_result_device_view(self.h, C.byref(dv)))
+        return dv
+
+    def keys(self):
+        n, m, _ = self.sizes()
+        k = np.zeros(n, np.uint64)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_keys(self.h, k.ctypes.data))
+        return k
+
+    def to_complex(self):
+        n, m, nconn = self.sizes()
+        nv = m + 1
+        verts = np.zeros((n, nv), np.uint64)
+        lam = np.zeros((n, nv), np.float64)
+        bary = np.zeros((n, nv), np.float32)
+        scal = np.zeros(n, np.float32)
+        track = np.zeros(n, np.uint64)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_nodes(self.h, verts.ctypes.data, lam.ctypes.data,
+                                                          bary.ctypes.data, scal.ctypes.data,
+                                                          track.ctypes.data))
+        conn = {}
+        for i in range(nconn):
+            dim = C.c_int()
+            cells = C.c_uint64()
+            _lib.check(_lib.lib().ftk2h_result_conn_info(self.h, i, C.byref(dim), C.byref(cells)))
+            buf = np.zeros(cells.value * (dim.value + 1), np.uint32)
+            if cells.value:
+                _lib.check(_lib.lib().ftk2h_result_copy_conn(self.h, i, buf.ctypes.data))
+            conn[dim.value] = buf.reshape(-1, dim.value + 1)
+        return FeatureComplex(verts, lam, bary, scal, track, conn)
+
+
+@dataclass
+class FeatureComplex:
+    vertices: np.ndarray      # (n, m+1) sorted vertex ids of each node's simplex
+    lam: np.ndarray           # (n, m+1) solver output (double)
+    bary: np.ndarray          # (n, m+1) FeatureElement::barycentric_coords[0] (float)
+    scalar: np.ndarray        # (n,) FeatureElement::scalar
+    track_id: np.ndarray      # (n,) minimum canonical index of the component
+    connectivity: dict        # dim -> (cells, dim+1) uint32, sorted, unique
+
+
+def extract(mesh: RegularSimplicialMesh, kind: int, m: int, comps, thresholds=(0.0, 0.0),
+            sos_q: float = 1e6, scalar=None, scalar_shift=None, device: int = 0, stream=None,
+            node_capacity: int = 0, cell_capacity: int = 0) -> Result:
+    """One ftk2h_extract_regular call.  comps: arrays indexed by local coordinates."""
+    comps = list(comps)
+    dt = _dtype_of(comps[0])
+    for c in comps:
+        if _dtype_of(c) != dt:
+            raise TypeError("all components must share one dtype")
+    views = (_lib.FieldView * len(comps))(*[field_view(c) for c in comps])
+    pd = _lib.PredicateDesc()
+    pd.kind = kind
+    pd.codimension = m
+    pd.dtype = dt
+    pd.sos_q = sos_q
+    pd.thresholds[0] = float(thresholds[0])
+    pd.thresholds[1] = float(thresholds[1]) if len(thresholds) > 1 else 0.0
+    sv = None
+    if scalar is not None:
+        sv = field_view(scalar, scalar_shift)
+    opt = _lib.Options(node_capacity, cell_capacity, 1)
+    out = C.c_void_p()
+    md = mesh.desc()
+    _lib.check(_lib.lib().ftk2h_extract_regular(
+        context(device), C.byref(md), C.byref(pd), views, len(comps),
+        C.byref(sv) if sv is not None else None, C.byref(opt), C.byref(out),
+        C.c_void_p(stream) if stream else None))
+    return Result(out.value, keep=(comps, scalar))
+
+
+class SimplicialEngine:
+    """execute()/get_complex() on a RegularSimplicialMesh (engine.hpp:79-99, 822)."""
+
+    def __init__(self, mesh: RegularSimplicialMesh, pred, device: int = 0):
+        self.mesh = mesh
+        self.pred = pred
+        self.device = device
+        self._result = None
+
+    def execute(self, data: dict, var_names=None):
+        p = self.pred
+        mesh = self.mesh
+        scalar = None
+        shift = None
+        if isinstance(p, CriticalPointPredicate):
+            if p.use_multicomponent and p.vector_var_name:
+                # [M, x, y, (z), t] read at GLOBAL coordinates (engine.hpp:1101-1124)
+                vec = data[p.vector_var_name]
+                comps = [vec[j] for j in range(p.M)]
+                comps = [_shifted(c, mesh.offset) for c in comps]
+            else:
+                comps = [data[n] for n in p.var_names[: p.M]]
+            if p.scalar_var_name:
+                scalar = data[p.scalar_var_name]
+                shift = mesh.offset  # predicate.hpp:84-89 reads global coordinates
+            thr = (0.0, 0.0)
+        elif isinstance(p, ContourPredicate):
+            comps = [data[p.var_name]]
+            thr = (p.threshold, 0.0)
+        elif isinstance(p, FiberPredicate):
+            comps = [data[p.var_names[0]], data[p.var_names[1]]]
+            thr = tuple(p.thresholds)
+        else:
+            raise TypeError("unsupported predicate")
+        self._result = extract(mesh, p.kind, p.codimension, comps, thr, p.sos_q, scalar, shift,
+                               device=self.device)
+
+    def get_complex(self) -> FeatureComplex:
+        if self._result is None:
+            raise RuntimeError("execute() first")
+        return self._result.to_complex()
+
+
+def _shifted(a, shift):
+    if not any(shift):
+        return a
+    sl = tuple(slice(int(o), None) for o in shift)
+    return a[sl]

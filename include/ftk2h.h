@@ -1,0 +1,138 @@
+/*
+ * ftk2h.h — C ABI of the MI355X (gfx950) simplicial extraction library.
+ *
+ * This is the device boundary behind FTK2's SimplicialEngine<T, Predicate>:
+ * include/ftk2/core/engine.hpp in this repository implements execute() and
+ * get_complex() on top of these entry points.  Plain C types only: pointers,
+ * sizes, status codes.  Every entry point returns an ftk2h_status; on failure
+ * ftk2h_last_error() holds a message for the calling thread.  Nothing here calls
+ * exit(); nothing here falls back to a CPU implementation.
+ *
+ * Reference interfaces replaced (hguo/ftk2, paths relative to its root):
+ *   ftk2h_extract_regular      SimplicialEngine::execute()
+ *                              include/ftk2/core/engine.hpp:96 (feed :101-212) and
+ *                              SimplicialEngine::execute_cuda_regular :302-425
+ *   ftk2h_result_*             SimplicialEngine::get_complex() engine.hpp:822-996,
+ *                              FeatureElement feature.hpp:22-39,
+ *                              FeatureComplex + sort() complex.hpp:14-58
+ *   ftk2h_predicate_desc       CriticalPointPredicate<M,T> predicate.hpp:49-130,
+ *                              ContourPredicate<T> :135-183, FiberPredicate<T> :340-389
+ *                              (fields sos_q :25, threshold(s), scalar_var_name)
+ *   ftk2h_mesh_desc            RegularSimplicialMesh(dims, offset, global_dims)
+ *                              include/ftk2/core/mesh.hpp:256-265
+ *   ftk2h_field_view           ftk::ndarray<T> read through get_value
+ *                              engine.hpp:999-1009 (hguo/ndarray, un-vendored)
+ *   ftk2h_stream_*             SimplicialEngine::execute_stream engine.hpp:214-270 and
+ *                              execute_cuda_streaming_regular :601-819
+ */
+#ifndef FTK2H_H
+#define FTK2H_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FTK2H_OK = 0,
+  FTK2H_E_INVALID = 1,     /* bad argument (null pointer, bad dims, bad kind) */
+  FTK2H_E_UNSUPPORTED = 2, /* (predicate, ndims, codimension, dtype) not built */
+  FTK2H_E_OVERFLOW = 3,    /* a fixed capacity in ftk2h_options was too small */
+  FTK2H_E_OOM = 4,         /* device allocation failed */
+  FTK2H_E_HIP = 5          /* a HIP runtime call failed */
+} ftk2h_status;
+
+typedef enum {
+  FTK2H_PRED_CRITICAL_POINT = 0, /* CriticalPointPredicate<M,T>, codimension M = 1..3 */
+  FTK2H_PRED_CONTOUR = 1,        /* ContourPredicate<T>, codimension 1 */
+  FTK2H_PRED_FIBER = 2           /* FiberPredicate<T>, codimension 2 */
+} ftk2h_pred_kind;
+
+typedef enum { FTK2H_F32 = 0, FTK2H_F64 = 1 } ftk2h_dtype;
+
+typedef struct ftk2h_ctx ftk2h_ctx;
+typedef struct ftk2h_result ftk2h_result;
+
+/* RegularSimplicialMesh: the last axis is time.  Vertex id =
+ * x + gx*(y + gy*(z + gz*t)) over global dims of (local + offset) coordinates. */
+typedef struct {
+  int ndims;               /* total dimension d = 2..4 */
+  uint64_t local_dims[4];  /* vertices per axis of this mesh (slab) */
+  uint64_t offset[4];      /* global coordinate of local (0,...,0) */
+  uint64_t global_dims[4]; /* extents used for vertex ids */
+} ftk2h_mesh_desc;
+
+typedef struct {
+  int kind;              /* ftk2h_pred_kind */
+  int codimension;       /* m */
+  int dtype;             /* ftk2h_dtype of every field view */
+  double sos_q;          /* Predicate::sos_q, default 1e6 */
+  double thresholds[2];  /* contour: [0]; fiber: [0],[1]; CP: unused */
+} ftk2h_predicate_desc;
+
+/* One scalar component indexed by LOCAL vertex coordinates:
+ * element(l) = ptr[sum_j l_j * strides[j]]  (strides in elements).
+ * The reference's legacy separate-array mode reads exactly this. */
+typedef struct {
+  const void* ptr;
+  int64_t strides[4];
+  int on_device; /* 1: device pointer read in place; 0: host memory, copied in */
+} ftk2h_field_view;
+
+typedef struct {
+  uint64_t node_capacity; /* 0: size from the data; else fail with E_OVERFLOW beyond */
+  uint64_t cell_capacity; /* same for connectivity cells */
+  int sync;               /* 1: wait for completion before returning (default behaviour
+                             when the struct is NULL) */
+} ftk2h_options;
+
+/* Device-resident view of a result (pointers stay owned by the result). */
+typedef struct {
+  uint64_t n_nodes;
+  int node_dim;              /* m */
+  const uint64_t* keys;      /* canonical node keys: v_min * n_types + type, sorted */
+  const void* lambda;        /* n_nodes * (m+1) values of the predicate dtype */
+  const uint32_t* track_id;  /* minimum canonical index of the node's component */
+  const float* scalar;       /* per node, or NULL */
+  int n_conn;
+  int conn_dim[3];
+  uint64_t conn_cells[3];
+  const uint32_t* conn[3];   /* conn_cells * (conn_dim+1) canonical indices, sorted */
+} ftk2h_device_view;
+
+int ftk2h_ctx_create(int device, ftk2h_ctx** out);
+void ftk2h_ctx_destroy(ftk2h_ctx* ctx);
+
+/* execute() + get_complex() on a RegularSimplicialMesh.  comps: ncomp views
+ * (CP: M components, contour: 1, fiber: 2).  scalar: optional CP scalar field
+ * interpolated at the feature (predicate.hpp:81-92), indexed like the reference
+ * (global coordinates: the caller passes a view whose local index equals the
+ * global one).  hip_stream: a hipStream_t or NULL for the context stream. */
+int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh,
+                          const ftk2h_predicate_desc* pred, const ftk2h_field_view* comps,
+                          int ncomp, const ftk2h_field_view* scalar,
+                          const ftk2h_options* opt, ftk2h_result** out, void* hip_stream);
+
+int ftk2h_result_sizes(const ftk2h_result* r, uint64_t* n_nodes, int* node_dim, int* n_conn);
+int ftk2h_result_conn_info(const ftk2h_result* r, int i, int* dim, uint64_t* n_cells);
+/* Host copies; any pointer may be NULL.  vertices: n*(m+1) sorted vertex ids (the
+ * Simplex of each FeatureElement); lambda: n*(m+1) doubles (solver output before
+ * the float cast); bary: n*(m+1) floats (FeatureElement::barycentric_coords[0]);
+ * scalar: n floats; track_id: n (FeatureElement::track_id). */
+int ftk2h_result_copy_nodes(const ftk2h_result* r, uint64_t* vertices, double* lambda,
+                            float* bary, float* scalar, uint64_t* track_id);
+int ftk2h_result_copy_keys(const ftk2h_result* r, uint64_t* keys);
+int ftk2h_result_copy_conn(const ftk2h_result* r, int i, uint32_t* indices);
+int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out);
+/* Per-phase device time in ms of the extraction that produced r (n entries,
+ * phases: classify, scan nodes, offsets+cell count, scan cells, emit, link, compress). */
+int ftk2h_result_phase_ms(const ftk2h_result* r, float* ms, int n);
+void ftk2h_result_free(ftk2h_result* r);
+
+const char* ftk2h_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FTK2H_H */

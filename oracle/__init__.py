@@ -1,0 +1,217 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper over ``liboracle.so``, the CPU restatement of FTK2's CPU engine
+(include/ftk2/core/engine.hpp execute()/get_complex() and the predicates it runs).
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
+may import this package, and only as the checker (or the timed CPU baseline).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+PRED_CP, PRED_CONTOUR, PRED_FIBER = 0, 1, 2
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        P, U64P, I64P, DP = C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        L.oracle_run.restype = P
+        L.oracle_run.argtypes = [C.c_int, U64P, U64P, U64P, C.c_int, C.c_int, C.c_int, C.c_double, DP,
+                                 C.c_int, C.POINTER(C.c_void_p), I64P, C.c_void_p, I64P, C.c_int]
+        L.oracle_run_stream.restype = P
+        L.oracle_run_stream.argtypes = [C.c_int, U64P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, DP,
+                                        C.c_int, C.POINTER(C.c_void_p), C.c_int]
+        L.oracle_error.restype = C.c_char_p
+        L.oracle_n_nodes.restype = C.c_uint64
+        L.oracle_n_nodes.argtypes = [P]
+        L.oracle_node_dim.argtypes = [P]
+        L.oracle_nodes.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_n_conn.argtypes = [P]
+        L.oracle_conn_dim.argtypes = [P, C.c_int]
+        L.oracle_conn_size.restype = C.c_uint64
+        L.oracle_conn_size.argtypes = [P, C.c_int]
+        L.oracle_conn.argtypes = [P, C.c_int, C.c_void_p]
+        L.oracle_n_components.restype = C.c_uint64
+        L.oracle_n_components.argtypes = [P]
+        L.oracle_free.argtypes = [P]
+        L.oracle_sign.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_double]
+        L.oracle_det2.argtypes = [C.c_int, DP, DP, C.c_uint64, C.c_uint64, C.c_double]
+        L.oracle_det3.argtypes = [C.c_int, DP, DP, DP, C.c_double]
+        L.oracle_det3_level0.argtypes = [I64P, I64P, I64P, I64P, U64P]
+        L.oracle_origin_inside.argtypes = [C.c_int, C.c_int, DP, U64P, C.c_double]
+        L.oracle_solve.argtypes = [C.c_int, C.c_int, DP, DP]
+        L.oracle_chains.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        L.oracle_encode_simplex_id.restype = C.c_uint64
+        L.oracle_encode_simplex_id.argtypes = [C.c_int, U64P, C.c_int, U64P]
+        _lib = L
+    return _lib
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _dp(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _collect(h):
+    L = lib()
+    if not h:
+        raise RuntimeError("oracle failed: " + L.oracle_error().decode())
+    try:
+        n = L.oracle_n_nodes(h)
+        nv = L.oracle_node_dim(h) + 1
+        verts = np.zeros((n, nv), np.uint64)
+        lam = np.zeros((n, nv), np.float64)
+        bary = np.zeros((n, nv), np.float32)
+        scal = np.zeros(n, np.float32)
+        track = np.zeros(n, np.uint64)
+        L.oracle_nodes(h, verts.ctypes.data, lam.ctypes.data, bary.ctypes.data, scal.ctypes.data,
+                       track.ctypes.data)
+        conn = {}
+        for i in range(L.oracle_n_conn(h)):
+            dim = L.oracle_conn_dim(h, i)
+            sz = L.oracle_conn_size(h, i)
+            buf = np.zeros(sz, np.uint32)
+            L.oracle_conn(h, i, buf.ctypes.data)
+            conn[dim] = buf.reshape(-1, dim + 1)
+        return {"verts": verts, "lam": lam, "bary": bary, "scalar": scal, "track": track,
+                "conn": conn, "n_components": int(L.oracle_n_components(h))}
+    finally:
+        L.oracle_free(h)
+
+
+def run(comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e6, scalar=None,
+        nthreads=0):
+    """Run the CPU engine on a regular mesh.
+
+    comps: list of numpy arrays (float32 or float64), indexed [x, y, (z), t] on local
+    coordinates, any strides (the reference's legacy separate-array mode).
+    """
+    comps = list(comps)
+    dtype = 0 if comps[0].dtype == np.float32 else 1
+    d = comps[0].ndim
+    if dims is None:
+        dims = comps[0].shape
+    item = comps[0].itemsize
+    ptrs = (C.c_void_p * len(comps))(*[c.ctypes.data for c in comps])
+    st = np.zeros(len(comps) * 4, np.int64)
+    for i, c in enumerate(comps):
+        assert c.dtype == comps[0].dtype
+        for j in range(d):
+            st[i * 4 + j] = c.strides[j] // item
+    dims_a, dims_p = _u64(list(dims) + [1] * (4 - d))
+    off_a, off_p = _u64((list(offset) if offset is not None else [0] * d) + [0] * (4 - d))
+    g_a, g_p = _u64((list(gdims) if gdims is not None else list(dims)) + [1] * (4 - d))
+    thr_a, thr_p = _dp(list(thr) + [0.0] * (2 - len(thr)))
+    sst = np.zeros(4, np.int64)
+    sptr = None
+    if scalar is not None:
+        assert scalar.dtype == comps[0].dtype
+        sptr = scalar.ctypes.data
+        for j in range(d):
+            sst[j] = scalar.strides[j] // item
+    h = lib().oracle_run(d, dims_p, off_p, g_p, kind, m, dtype, Q, thr_p, len(comps), ptrs,
+                         st.ctypes.data_as(C.POINTER(C.c_int64)), sptr,
+                         sst.ctypes.data_as(C.POINTER(C.c_int64)), nthreads)
+    return _collect(h)
+
+
+def run_stream(steps, kind, m, thr=(0.0, 0.0), Q=1e6, nthreads=0):
+    """execute_stream(): steps[t] is a list of per-component spatial arrays (x fastest)."""
+    nt = len(steps)
+    ncomp = len(steps[0])
+    sp = steps[0][0].shape
+    dtype = 0 if steps[0][0].dtype == np.float32 else 1
+    flat = []
+    for t in range(nt):
+        for c in range(ncomp):
+            a = steps[t][c]
+            assert a.flags.f_contiguous and a.shape == sp
+            flat.append(a)
+    ptrs = (C.c_void_p * len(flat))(*[a.ctypes.data for a in flat])
+    sp_a, sp_p = _u64(list(sp))
+    thr_a, thr_p = _dp(list(thr) + [0.0] * (2 - len(thr)))
+    h = lib().oracle_run_stream(len(sp), sp_p, nt, kind, m, dtype, Q, thr_p, ncomp, ptrs, nthreads)
+    return _collect(h)
+
+
+# ---- unit-level helpers -------------------------------------------------------
+def sign(v, idx, Q=1e6, dtype=1):
+    return lib().oracle_sign(dtype, float(v), int(idx), Q)
+
+
+def det2(a, b, i0, i1, Q=1e6, dtype=1):
+    a_, ap = _dp(a)
+    b_, bp = _dp(b)
+    return lib().oracle_det2(dtype, ap, bp, int(i0), int(i1), Q)
+
+
+def det3(a, b, c, Q=1e6, dtype=1):
+    a_, ap = _dp(a)
+    b_, bp = _dp(b)
+    c_, cp = _dp(c)
+    return lib().oracle_det3(dtype, ap, bp, cp, Q)
+
+
+def det3_level0(qa, qb, qc):
+    arrs = [np.ascontiguousarray(x, dtype=np.int64) for x in (qa, qb, qc)]
+    hi = C.c_int64()
+    lo = C.c_uint64()
+    P = C.POINTER(C.c_int64)
+    lib().oracle_det3_level0(arrs[0].ctypes.data_as(P), arrs[1].ctypes.data_as(P),
+                             arrs[2].ctypes.data_as(P), C.byref(hi), C.byref(lo))
+    return (hi.value << 64) + lo.value
+
+
+def origin_inside(values, idx, Q=1e6, dtype=1):
+    v = np.asarray(values, dtype=np.float64)
+    K = v.shape[1]
+    v_, vp = _dp(v.ravel())
+    i_, ip = _u64(idx)
+    return bool(lib().oracle_origin_inside(K, dtype, vp, ip, Q))
+
+
+def solve(values, dtype=1):
+    v = np.asarray(values, dtype=np.float64)
+    K = v.shape[1]
+    v_, vp = _dp(v.ravel())
+    lam = np.zeros(K + 1, np.float64)
+    ok = lib().oracle_solve(K, dtype, vp, lam.ctypes.data_as(C.POINTER(C.c_double)))
+    return bool(ok), lam
+
+
+def chains(d, m):
+    out = np.zeros(64 * 5, np.uint8)
+    n = lib().oracle_chains(d, m, out.ctypes.data)
+    return out[: n * (m + 1)].reshape(n, m + 1)
+
+
+def encode_simplex_id(gdims, verts):
+    d = len(gdims)
+    g_a, g_p = _u64(list(gdims))
+    v_a, v_p = _u64(list(verts))
+    return int(lib().oracle_encode_simplex_id(d, g_p, len(verts) - 1, v_p))
