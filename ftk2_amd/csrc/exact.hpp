@@ -78,6 +78,45 @@ __device__ __noinline__ int det3_levels(const int64_t* X0, const int64_t* X1, co
   return 1;  // t=14
 }
 
+// det3 levels in 64-bit integers: exact when every |q| < 2^20 (level 0 terms stay
+// below 3 * 2^61; minors with the ones column below 2^43).
+__device__ __forceinline__ int sgn64(int64_t x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+__device__ __forceinline__ int64_t det3_ones64(int64_t a0, int64_t a1, int64_t b0, int64_t b1) {
+  // [[a0,a1,1],[b0,b1,1],[0,0,1]] (third row is the origin)
+  return a0 * b1 - a1 * b0;
+}
+__device__ __forceinline__ int det3_levels64(const int64_t* X0, const int64_t* X1, const int64_t* X2) {
+  const int64_t a0 = X0[0], a1 = X0[1], a2 = X0[2];
+  const int64_t b0 = X1[0], b1 = X1[1], b2 = X1[2];
+  const int64_t c0 = X2[0], c1 = X2[1], c2 = X2[2];
+  int64_t d = a0 * (b1 * c2 - b2 * c1) - a1 * (b0 * c2 - b2 * c0) + a2 * (b0 * c1 - b1 * c0);
+  if (d) return sgn64(d);
+  if ((d = det3_ones64(b0, b1, c0, c1))) return sgn64(d);    // t=1
+  if ((d = -det3_ones64(b0, b2, c0, c2))) return sgn64(d);   // t=2
+  if ((d = det3_ones64(b1, b2, c1, c2))) return sgn64(d);    // t=3
+  if ((d = -det3_ones64(a0, a1, c0, c1))) return sgn64(d);   // t=4
+  if ((d = c0)) return sgn64(d);                             // t=5
+  if ((d = -c1)) return sgn64(d);                            // t=6
+  if ((d = det3_ones64(a0, a2, c0, c2))) return sgn64(d);    // t=7
+  if ((d = c2)) return sgn64(d);                             // t=8
+  if ((d = -det3_ones64(a1, a2, c1, c2))) return sgn64(d);   // t=9
+  if ((d = det3_ones64(a0, a1, b0, b1))) return sgn64(d);    // t=10
+  if ((d = -b0)) return sgn64(d);                            // t=11
+  if ((d = b1)) return sgn64(d);                             // t=12
+  if ((d = a0)) return sgn64(d);                             // t=13
+  return 1;                                                  // t=14
+}
+
+// origin_inside<3> in 64-bit arithmetic (all |q| < 2^20).
+__device__ __forceinline__ bool inside3_exact64(const int64_t (*X)[3]) {
+  const int d0 = det3_levels64(X[1], X[3], X[2]);
+  const int d1 = det3_levels64(X[0], X[2], X[3]);
+  if (d0 != d1) return false;
+  const int d2 = det3_levels64(X[0], X[3], X[1]);
+  if (d1 != d2) return false;
+  return d2 == det3_levels64(X[0], X[1], X[2]);
+}
+
 // origin_inside<3> (sos.hpp:234-254) on quantized vertices in ascending id order.
 __device__ __noinline__ bool inside3_exact(const int64_t (*X)[3]) {
   const int d0 = det3_levels(X[1], X[3], X[2]);

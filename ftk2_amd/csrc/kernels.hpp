@@ -203,38 +203,22 @@ __device__ __forceinline__ int dot_sign(const double* e, const float* C, double 
 }
 
 // ---------------------------------------------------------------------------
+// Per-vertex classification, two stages.
+//
+// Stage 1 (classify_fast, fully unrolled, branch-light): for every simplex type the
+// vertex owns, decide from exact level-0 signs whether it is rejected, inside by
+// SoS (needs the solver gate for critical points), or degenerate (some level-0
+// value is exactly zero, or the magnitudes exceed the fast filter: needs the
+// verbatim multi-level sign tests).  Stage 2 runs those rare per-simplex jobs from
+// a workgroup queue in LDS so every lane does useful work (eval_item).
+//
+// Bits: acc = accepted, sol = SoS inside, solver pending, deg = verbatim pending.
+
 // Critical points of a 3-component field on 3-simplices (CriticalPointPredicate<3,T>,
 // predicate.hpp:66-99).
-
 template <class T, int D>
-__device__ __noinline__ bool cp3_full(const KParams& P, const int64_t* l, int a, int b, int c) {
-  const int off[4] = {0, a, b, c};
-  T vals[4][3];
-  int64_t X[4][3];
-  for (int i = 0; i < 4; ++i)
-    for (int k = 0; k < 3; ++k) {
-      vals[i][k] = load_val<T, D>(P, k, l, off[i]);
-      X[i][k] = q64(vals[i][k], P.Q);
-    }
-  if (!inside3_exact(X)) return false;
-  T lam[4];
-  return solve_k<3, T>(vals, lam);
-}
-
-template <class T, int D>
-__device__ __forceinline__ bool cp3_solve_ok(const KParams& P, const int64_t* l, int a, int b, int c) {
-  const int off[4] = {0, a, b, c};
-  T vals[4][3];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vals[i][k] = load_val<T, D>(P, k, l, off[i]);
-  T lam[4];
-  return solve_k<3, T>(vals, lam);
-}
-
-template <class T, int D>
-__device__ __forceinline__ uint64_t classify_cp3(const KParams& P, const int64_t* l) {
+__device__ __forceinline__ void classify_cp3(const KParams& P, const int64_t* l, uint64_t& acc,
+                                             uint64_t& sol, uint64_t& deg) {
   constexpr int NC = 1 << D;
   const uint32_t inb = corner_mask<D>(P, l);
   float X[NC][3];
@@ -250,16 +234,14 @@ __device__ __forceinline__ uint64_t classify_cp3(const KParams& P, const int64_t
       X[s][k] = (float)q;
     }
   }
-  uint64_t mask = 0;
+  acc = sol = deg = 0;
   if (mx >= (int64_t(1) << 20)) {
-    // magnitudes beyond the fast filter: verbatim exact path for every tet
+    // magnitudes beyond the fast filter: every tet takes the verbatim path
     sfor<0, CH<D, 3>.n>([&](auto tt) {
       constexpr int t = decltype(tt)::value;
-      constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
-      if ((inb >> c) & 1)
-        if (cp3_full<T, D>(P, l, a, b, c)) mask |= 1ull << t;
+      if ((inb >> CH<D, 3>.s[t][3]) & 1) deg |= 1ull << t;
     });
-    return mask;
+    return;
   }
   const double md = (double)mx;
   const double bnd = md * md * md * 0x1p-48;
@@ -284,51 +266,28 @@ __device__ __forceinline__ uint64_t classify_cp3(const KParams& P, const int64_t
       // d3 = det3(v0,v1,v2) = tp(0ab); d2 = det3(v0,v3,v1) = -tp(0ac);
       // d1 = det3(v0,v2,v3) = tp(0bc); d0 = det3(v1,v3,v2) = -tp(abc)
       const int p012 = (tpos >> i012) & 1, p013 = (tpos >> i013) & 1, p023 = (tpos >> i023) & 1;
-      bool acc = false;
       if (z) {
-        acc = cp3_full<T, D>(P, l, a, b, c);
+        deg |= 1ull << t;
       } else if (p012 != p013 && p023 == p012) {
         double e[3];
         cross3(X[b], X[c], e);
         const int sabc = dot_sign(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
-        if (sabc == 0) {
-          acc = cp3_full<T, D>(P, l, a, b, c);
-        } else {
-          const int d0 = -sabc, d3 = p012 ? 1 : -1;
-          if (d0 == d3) acc = cp3_solve_ok<T, D>(P, l, a, b, c);
-        }
+        if (sabc == 0)
+          deg |= 1ull << t;
+        else if (-sabc == (p012 ? 1 : -1))
+          sol |= 1ull << t;
       }
-      if (acc) mask |= 1ull << t;
     }
   });
-  return mask;
 }
 
-// ---------------------------------------------------------------------------
 // det2-based predicates on 2-simplices: CriticalPointPredicate<2,T> and
 // FiberPredicate<T> (values minus thresholds, predicate.hpp:352).
-
 template <int KIND, class T, int D>
 __device__ __forceinline__ T val2(const KParams& P, int k, const int64_t* l, int s) {
   const T v = load_val<T, D>(P, k, l, s);
   if constexpr (KIND == KIND_FIBER) return v - (T)P.thr[k];
   return v;
-}
-
-template <int KIND, class T, int D>
-__device__ __noinline__ bool tri_full(const KParams& P, const int64_t* l, int a, int b) {
-  const int off[3] = {0, a, b};
-  T vals[3][2];
-  int64_t X[3][2];
-  for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 2; ++k) {
-      vals[i][k] = val2<KIND, T, D>(P, k, l, off[i]);
-      X[i][k] = q64(vals[i][k], P.Q);
-    }
-  if (!inside2_exact(X)) return false;
-  if constexpr (KIND == KIND_FIBER) return true;  // solver failure falls back to 1/3
-  T lam[3];
-  return solve_k<2, T>(vals, lam);
 }
 
 __device__ __forceinline__ double cross2(const double* A, const double* B) {
@@ -337,7 +296,8 @@ __device__ __forceinline__ double cross2(const double* A, const double* B) {
 }
 
 template <int KIND, class T, int D>
-__device__ __forceinline__ uint64_t classify_tri(const KParams& P, const int64_t* l) {
+__device__ __forceinline__ void classify_tri(const KParams& P, const int64_t* l, uint64_t& acc,
+                                             uint64_t& sol, uint64_t& deg) {
   constexpr int NC = 1 << D;
   const uint32_t inb = corner_mask<D>(P, l);
   double X[NC][2];
@@ -353,15 +313,14 @@ __device__ __forceinline__ uint64_t classify_tri(const KParams& P, const int64_t
       X[s][k] = (double)q;
     }
   }
+  acc = sol = deg = 0;
   const bool slow = mx >= (int64_t(1) << 26);
-  uint64_t mask = 0;
   sfor<0, CH<D, 2>.n>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
     if ((inb >> b) & 1) {
-      bool acc;
       if (slow) {
-        acc = tri_full<KIND, T, D>(P, l, a, b);
+        deg |= 1ull << t;
       } else {
         const double c01 = cross2(X[0], X[a]);
         const double c12 = cross2(X[a], X[b]);
@@ -369,68 +328,98 @@ __device__ __forceinline__ uint64_t classify_tri(const KParams& P, const int64_t
         const int s0 = c01 < 0 ? -1 : 1;  // tie: i0 < i1 -> +1
         const int s1 = c12 < 0 ? -1 : 1;  // tie: i1 < i2 -> +1
         const int s2 = c20 > 0 ? 1 : -1;  // tie: i2 > i0 -> -1
-        acc = (s0 == s1) && (s1 == s2);
-        if constexpr (KIND == KIND_CP) {
-          if (acc) {
-            T vals[3][2];
-            const int off[3] = {0, a, b};
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-              for (int k = 0; k < 2; ++k) vals[i][k] = load_val<T, D>(P, k, l, off[i]);
-            T lam[3];
-            acc = solve_k<2, T>(vals, lam);
-          }
+        if (s0 == s1 && s1 == s2) {
+          if constexpr (KIND == KIND_CP)
+            sol |= 1ull << t;
+          else
+            acc |= 1ull << t;  // fiber: solver failure falls back to 1/3
         }
       }
-      if (acc) mask |= 1ull << t;
     }
   });
-  return mask;
 }
 
-// ---------------------------------------------------------------------------
 // Edge predicates: ContourPredicate<T> (sign change of value - threshold) and
 // CriticalPointPredicate<1,T> (sign change + solver gate).
 template <int KIND, class T, int D>
-__device__ __forceinline__ uint64_t classify_edge(const KParams& P, const int64_t* l) {
+__device__ __forceinline__ void classify_edge(const KParams& P, const int64_t* l, uint64_t& acc,
+                                              uint64_t& sol, uint64_t& deg) {
   constexpr int NC = 1 << D;
   const uint32_t inb = corner_mask<D>(P, l);
   int sg[NC];
-  T v[NC];
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
-    v[s] = T(0);
     sg[s] = 1;
     if ((inb >> s) & 1) {
-      v[s] = load_val<T, D>(P, 0, l, s);
-      if constexpr (KIND == KIND_CONTOUR) v[s] = v[s] - (T)P.thr[0];
-      sg[s] = sos_sign(v[s], gid_of<D>(P, l, s), P.Q);
+      T v = load_val<T, D>(P, 0, l, s);
+      if constexpr (KIND == KIND_CONTOUR) v = v - (T)P.thr[0];
+      sg[s] = sos_sign(v, gid_of<D>(P, l, s), P.Q);
     }
   }
-  uint64_t mask = 0;
+  acc = sol = deg = 0;
 #pragma unroll
   for (int s = 1; s < NC; ++s) {
     if (((inb >> s) & 1) && sg[s] != sg[0]) {
-      bool acc = true;
-      if constexpr (KIND == KIND_CP) {
-        T lam[2];
-        acc = solve_1<T>(v[0], v[s], lam);
-      }
-      if (acc) mask |= 1ull << (s - 1);  // Chains<D,1> type of (0, s) is s - 1
+      // Chains<D,1> type of edge (0, s) is s - 1
+      if constexpr (KIND == KIND_CP)
+        sol |= 1ull << (s - 1);
+      else
+        acc |= 1ull << (s - 1);
     }
   }
-  return mask;
 }
 
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ uint64_t classify_vertex(const KParams& P, const int64_t* l) {
-  if constexpr (M == 3) {
-    return classify_cp3<T, D>(P, l);
-  } else if constexpr (M == 2) {
-    return classify_tri<KIND, T, D>(P, l);
+__device__ __forceinline__ void classify_fast(const KParams& P, const int64_t* l, uint64_t& acc,
+                                              uint64_t& sol, uint64_t& deg) {
+  if constexpr (M == 3)
+    classify_cp3<T, D>(P, l, acc, sol, deg);
+  else if constexpr (M == 2)
+    classify_tri<KIND, T, D>(P, l, acc, sol, deg);
+  else
+    classify_edge<KIND, T, D>(P, l, acc, sol, deg);
+}
+
+// Stage 2: one pending simplex (type t owned by the vertex at l).  deg: re-run the
+// verbatim sign tests; then the solver gate for critical points.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ bool eval_item(const KParams& P, const int64_t* l, int t, bool deg) {
+  const uint32_t pc = packed_chain<D, M>(t);
+  int off[M + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
+  if constexpr (M == 1) {
+    T v0 = load_val<T, D>(P, 0, l, off[0]);
+    T v1 = load_val<T, D>(P, 0, l, off[1]);
+    T lam[2];
+    return solve_1<T>(v0, v1, lam);  // only CriticalPointPredicate<1> queues edges
   } else {
-    return classify_edge<KIND, T, D>(P, l);
+    T vals[M + 1][M];
+    int64_t X[M + 1][M];
+    int64_t mx = 0;
+#pragma unroll
+    for (int i = 0; i <= M; ++i)
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        T v = load_val<T, D>(P, k, l, off[i]);
+        if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
+        vals[i][k] = v;
+        X[i][k] = q64(v, P.Q);
+        const int64_t aq = X[i][k] < 0 ? -X[i][k] : X[i][k];
+        mx = aq > mx ? aq : mx;
+      }
+    if (deg) {
+      bool inside;
+      if constexpr (M == 3)
+        inside = mx < (int64_t(1) << 20) ? inside3_exact64(X) : inside3_exact(X);
+      else
+        inside = inside2_exact(X);
+      if (!inside) return false;
+    }
+    if constexpr (KIND == KIND_FIBER) return true;
+    T lam[M + 1];
+    return solve_k<M, T>(vals, lam);
   }
 }
 
@@ -775,14 +764,42 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const int64_t* l,
 // Kernels
 template <int KIND, int D, int M, class T>
 __global__ __launch_bounds__(BLOCK) void k_classify(KParams P) {
-  const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  uint64_t m = 0;
+  __shared__ uint16_t queue[BLOCK * 64];
+  __shared__ unsigned long long slot[BLOCK];
+  __shared__ uint32_t qn;
+  if (threadIdx.x == 0) qn = 0;
+  __syncthreads();
+  const int64_t base_v = (int64_t)blockIdx.x * BLOCK;
+  const int64_t v = base_v + threadIdx.x;
+  uint64_t acc = 0, sol = 0, deg = 0;
   if (v < P.nvert) {
     int64_t l[4];
     decompose<D>(P, v, l);
-    m = classify_vertex<KIND, D, M, T>(P, l);
-    P.mask[v] = m;
+    classify_fast<KIND, D, M, T>(P, l, acc, sol, deg);
   }
+  slot[threadIdx.x] = acc;
+  uint64_t w = sol | deg;
+  if (w) {
+    uint32_t pos = atomicAdd(&qn, (uint32_t)__popcll(w));
+    while (w) {
+      const int t = __builtin_ctzll(w);
+      w &= w - 1;
+      queue[pos++] = (uint16_t)(threadIdx.x | (t << 8) | (((deg >> t) & 1) << 14));
+    }
+  }
+  __syncthreads();
+  const uint32_t n = qn;
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+    const uint32_t it = queue[i];
+    const int owner = it & 0xFF, t = (it >> 8) & 0x3F;
+    const bool dg = (it >> 14) & 1;
+    int64_t l[4];
+    decompose<D>(P, base_v + owner, l);
+    if (eval_item<KIND, D, M, T>(P, l, t, dg)) atomicOr(&slot[owner], 1ull << t);
+  }
+  __syncthreads();
+  const uint64_t m = slot[threadIdx.x];
+  if (v < P.nvert) P.mask[v] = m;
   uint32_t tot;
   block_excl_scan((uint32_t)__popcll(m), &tot);
   if (threadIdx.x == 0) P.bsum[blockIdx.x] = tot;
