@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -76,6 +78,13 @@ def main():
     # index [x, y, z, t] views of the device arrays (x fastest in memory)
     dev_views = [d.permute(3, 2, 1, 0) for d in dev]
     torch.cuda.synchronize()
+    if args.calib_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        dst.copy_(src)  # reads and writes exactly 1 GiB each
+        torch.cuda.synchronize()
+        del src, dst
     mesh = engine.RegularSimplicialMesh([n, n, n, nt])
 
     def step():

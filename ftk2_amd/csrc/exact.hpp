@@ -28,6 +28,13 @@ __device__ __forceinline__ double qd(T v, double Q) {
   return __dadd_rn(prod, v > T(0) ? 0.5 : -0.5);
 }
 
+// The quantized value as an exact integer-valued double: one truncation instead
+// of a conversion to a 64-bit integer.
+template <class T>
+__device__ __forceinline__ double qt(T v, double Q) {
+  return trunc(qd(v, Q));
+}
+
 template <class T>
 __device__ __forceinline__ int64_t q64(T v, double Q) {
   return (int64_t)qd(v, Q);  // truncation; inputs with |q| >= 2^62 are out of domain
@@ -117,6 +124,57 @@ __device__ __forceinline__ bool inside3_exact64(const int64_t (*X)[3]) {
   return d2 == det3_levels64(X[0], X[1], X[2]);
 }
 
+// det3 levels on integer-valued doubles with every |q| < 2^20.  Level 0 (the
+// triple product, up to 6 * 2^60) is decided by an error-bounded FP64 evaluation,
+// falling back to exact 64-bit integers inside the bound; levels 1-13 are minors
+// below 2^43 and therefore exact in FP64.
+__device__ __forceinline__ int tp_sign_d(const double* A, const double* B, const double* C, double bnd) {
+  // A . (B x C); B x C is exact (products < 2^40)
+  const double e0 = fma(B[1], C[2], -(B[2] * C[1]));
+  const double e1 = fma(B[2], C[0], -(B[0] * C[2]));
+  const double e2 = fma(B[0], C[1], -(B[1] * C[0]));
+  double f = e0 * A[0];
+  f = fma(e1, A[1], f);
+  f = fma(e2, A[2], f);
+  if (f > bnd) return 1;
+  if (f < -bnd) return -1;
+  const int64_t s = (int64_t)e0 * (int64_t)A[0] + (int64_t)e1 * (int64_t)A[1] + (int64_t)e2 * (int64_t)A[2];
+  return s > 0 ? 1 : (s < 0 ? -1 : 0);
+}
+__device__ __forceinline__ int sgnd(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+__device__ __forceinline__ int det3_levels_d(const double* X0, const double* X1, const double* X2, double bnd) {
+  const int s0 = tp_sign_d(X0, X1, X2, bnd);
+  if (s0) return s0;
+  const double a0 = X0[0], a1 = X0[1], a2 = X0[2];
+  const double b0 = X1[0], b1 = X1[1], b2 = X1[2];
+  const double c0 = X2[0], c1 = X2[1], c2 = X2[2];
+  double d;
+  if ((d = fma(b0, c1, -(b1 * c0))) != 0) return sgnd(d);     // t=1
+  if ((d = -fma(b0, c2, -(b2 * c0))) != 0) return sgnd(d);    // t=2
+  if ((d = fma(b1, c2, -(b2 * c1))) != 0) return sgnd(d);     // t=3
+  if ((d = -fma(a0, c1, -(a1 * c0))) != 0) return sgnd(d);    // t=4
+  if ((d = c0) != 0) return sgnd(d);                          // t=5
+  if ((d = -c1) != 0) return sgnd(d);                         // t=6
+  if ((d = fma(a0, c2, -(a2 * c0))) != 0) return sgnd(d);     // t=7
+  if ((d = c2) != 0) return sgnd(d);                          // t=8
+  if ((d = -fma(a1, c2, -(a2 * c1))) != 0) return sgnd(d);    // t=9
+  if ((d = fma(a0, b1, -(a1 * b0))) != 0) return sgnd(d);     // t=10
+  if ((d = -b0) != 0) return sgnd(d);                         // t=11
+  if ((d = b1) != 0) return sgnd(d);                          // t=12
+  if ((d = a0) != 0) return sgnd(d);                          // t=13
+  return 1;                                                   // t=14
+}
+// origin_inside<3> on integer-valued doubles, all |q| < 2^20 (mx = max |q|).
+__device__ __forceinline__ bool inside3_exact_d(const double (*X)[3], double mx) {
+  const double bnd = mx * mx * mx * 0x1p-48;
+  const int d0 = det3_levels_d(X[1], X[3], X[2], bnd);
+  const int d1 = det3_levels_d(X[0], X[2], X[3], bnd);
+  if (d0 != d1) return false;
+  const int d2 = det3_levels_d(X[0], X[3], X[1], bnd);
+  if (d1 != d2) return false;
+  return d2 == det3_levels_d(X[0], X[1], X[2], bnd);
+}
+
 // origin_inside<3> (sos.hpp:234-254) on quantized vertices in ascending id order.
 __device__ __noinline__ bool inside3_exact(const int64_t (*X)[3]) {
   const int d0 = det3_levels(X[1], X[3], X[2]);
@@ -145,9 +203,10 @@ __device__ __forceinline__ bool inside2_exact(const int64_t (*X)[2]) {
 // sign (sos.hpp:12-20)
 template <class T>
 __device__ __forceinline__ int sos_sign(T v, uint64_t id, double Q) {
-  const int64_t q = q64(v, Q);
-  if (q > 0) return 1;
-  if (q < 0) return -1;
+  // trunc(r) > 0 <=> r >= 1; trunc(r) < 0 <=> r <= -1
+  const double r = qd(v, Q);
+  if (r >= 1.0) return 1;
+  if (r <= -1.0) return -1;
   return (id % 2 == 0) ? 1 : -1;
 }
 

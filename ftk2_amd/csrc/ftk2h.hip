@@ -2,6 +2,8 @@
 // objects and the C ABI declared in include/ftk2h.h.
 #include "internal.hpp"
 
+#include <cstdlib>
+
 using namespace ftk2h;
 using namespace ftk2h_internal;
 
@@ -247,6 +249,45 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
     if (rc) return rc;
   }
 
+  // per-corner offset tables and the packed-layout flag
+  {
+    int64_t lst[4] = {0, 0, 0, 0}, ls = 1;
+    for (int j = 0; j < d; ++j) {
+      lst[j] = ls;
+      ls *= (int64_t)mesh->local_dims[j];
+    }
+    P.packed = 1;
+    for (int c = 0; c < ncomp; ++c)
+      for (int j = 0; j < d; ++j)
+        if (P.cst[c][j] != lst[j]) P.packed = 0;
+    for (int s = 0; s < 16; ++s) {
+      int64_t lo = 0, so = 0;
+      uint64_t go = 0;
+      int64_t co[3] = {0, 0, 0};
+      for (int j = 0; j < d; ++j) {
+        if (!((s >> j) & 1)) continue;
+        lo += lst[j];
+        go += (uint64_t)P.gstride[j];
+        so += P.sst[j];
+        for (int c = 0; c < 3; ++c) co[c] += P.cst[c][j];
+      }
+      P.loff[s] = lo;
+      P.goff[s] = go;
+      P.soff[s] = so;
+      for (int c = 0; c < 3; ++c) P.coff[c][s] = co[c];
+    }
+  }
+  // Workgroup staging capacities of the emit kernel.  FTK2H_EMIT_CAPS="nq,words"
+  // lowers them (tests use it to drive the overflow paths).
+  P.nq_cap = EMIT_NQ;
+  P.stage_cap = EMIT_STAGE_WORDS;
+  if (const char* caps = std::getenv("FTK2H_EMIT_CAPS")) {
+    unsigned a = 0, b = 0;
+    if (std::sscanf(caps, "%u,%u", &a, &b) == 2) {
+      P.nq_cap = a < (unsigned)EMIT_NQ ? a : EMIT_NQ;
+      P.stage_cap = b < (unsigned)EMIT_STAGE_WORDS ? b : EMIT_STAGE_WORDS;
+    }
+  }
   const int64_t nb = (int64_t)((nvert + BLOCK - 1) / BLOCK);
   int rc;
   if ((rc = ctx->mask.ensure(nvert * 8))) return rc;

@@ -46,6 +46,14 @@ struct KParams {
   int64_t sst[4];
   double Q;
   double thr[2];
+  // per-corner offsets (corner s = bit mask of raised axes), precomputed on the host
+  int64_t coff[3][16];  // element offset of component c
+  int64_t soff[16];     // element offset of the scalar field
+  int64_t loff[16];     // local linear vertex offset
+  uint64_t goff[16];    // global vertex id offset
+  int packed;           // every component laid out x-fastest with local extents
+  uint32_t nq_cap;      // emit: queued nodes per workgroup (<= EMIT_NQ)
+  uint32_t stage_cap;   // emit: staged cell words per workgroup (<= EMIT_STAGE_WORDS)
   uint64_t* mask;
   uint32_t* node_off;
   uint32_t* bsum;   // per-block node counts -> exclusive offsets (+ total at [nb])
@@ -89,24 +97,55 @@ __device__ __forceinline__ void decompose(const KParams& P, int64_t v, int64_t* 
   }
 }
 
-// local linear index of l + s
+// A vertex as the kernels see it: local linear index and coordinates, element
+// index of every field component, global id.  The part a queued job needs is VtxRef.
+struct VtxRef {
+  int64_t base[3];
+  int64_t sbase;
+  uint64_t gid;
+};
+struct Vtx {
+  int64_t v;
+  int64_t l[4];
+  VtxRef r;
+};
+
 template <int D>
-__device__ __forceinline__ int64_t lin_of(const KParams& P, const int64_t* l, int s) {
-  int64_t r = 0, mul = 1;
+__device__ __forceinline__ VtxRef make_ref(const KParams& P, int64_t v, const int64_t* l) {
+  VtxRef r;
+  uint64_t g = 0;
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    r += (l[j] + ((s >> j) & 1)) * mul;
-    mul *= P.dims[j];
+  for (int j = 0; j < D; ++j) g += (uint64_t)(l[j] + P.off[j]) * (uint64_t)P.gstride[j];
+  r.gid = g;
+  if (P.packed) {
+    r.base[0] = r.base[1] = r.base[2] = v;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      int64_t e = 0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) e += l[j] * P.cst[c][j];
+      r.base[c] = e;
+    }
   }
+  int64_t e = 0;
+  if (P.scal) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) e += l[j] * P.sst[j];
+  }
+  r.sbase = e;
   return r;
 }
 
 template <int D>
-__device__ __forceinline__ uint64_t gid_of(const KParams& P, const int64_t* l, int s) {
-  uint64_t r = 0;
+__device__ __forceinline__ Vtx make_vtx(const KParams& P, int64_t v) {
+  Vtx x;
+  x.v = v;
+  decompose<D>(P, v, x.l);
 #pragma unroll
-  for (int j = 0; j < D; ++j) r += (uint64_t)(l[j] + ((s >> j) & 1) + P.off[j]) * (uint64_t)P.gstride[j];
-  return r;
+  for (int j = D; j < 4; ++j) x.l[j] = 0;
+  x.r = make_ref<D>(P, v, x.l);
+  return x;
 }
 
 // bit s set <=> corner l + s lies inside the local grid
@@ -124,22 +163,14 @@ __device__ __forceinline__ uint32_t corner_mask(const KParams& P, const int64_t*
   return m;
 }
 
-template <class T, int D>
-__device__ __forceinline__ T load_val(const KParams& P, int c, const int64_t* l, int s) {
-  const T* p = static_cast<const T*>(P.comp[c]);
-  int64_t idx = 0;
-#pragma unroll
-  for (int j = 0; j < D; ++j) idx += (l[j] + ((s >> j) & 1)) * P.cst[c][j];
-  return p[idx];
+template <class T>
+__device__ __forceinline__ T load_val(const KParams& P, int c, const VtxRef& r, int s) {
+  return static_cast<const T*>(P.comp[c])[r.base[c] + P.coff[c][s]];
 }
 
-template <class T, int D>
-__device__ __forceinline__ T load_scalar(const KParams& P, const int64_t* l, int s) {
-  const T* p = static_cast<const T*>(P.scal);
-  int64_t idx = 0;
-#pragma unroll
-  for (int j = 0; j < D; ++j) idx += (l[j] + ((s >> j) & 1)) * P.sst[j];
-  return p[idx];
+template <class T>
+__device__ __forceinline__ T load_scalar(const KParams& P, const VtxRef& r, int s) {
+  return static_cast<const T*>(P.scal)[r.sbase + P.soff[s]];
 }
 
 // chains packed 4 bits per vertex offset, for run-time type lookups
@@ -189,18 +220,46 @@ __device__ __forceinline__ void cross3(const float* A, const float* B, double* e
   e[2] = fma((double)A[0], (double)B[1], -((double)A[1] * (double)B[0]));
 }
 
-// sign of e . C where e = A x B (so the value is det[A; B; C])
+// Filtered sign of e . C where e = A x B (the value det[A; B; C]).  Returns +1/-1
+// when the FP64 value exceeds the rounding-error bound, 0 when it does not (the
+// caller then takes the exact path, which also covers exact zeros).
 __device__ __forceinline__ int dot_sign(const double* e, const float* C, double bnd) {
   double f = e[0] * (double)C[0];
   f = fma(e[1], (double)C[1], f);
   f = fma(e[2], (double)C[2], f);
-  if (f > bnd) return 1;
-  if (f < -bnd) return -1;
-  // |value| <= 3 * 2^41 * 2^20 < 2^63: exact in int64
-  const int64_t s = (int64_t)e[0] * (int64_t)C[0] + (int64_t)e[1] * (int64_t)C[1] +
-                    (int64_t)e[2] * (int64_t)C[2];
-  return s > 0 ? 1 : (s < 0 ? -1 : 0);
+  return f > bnd ? 1 : (f < -bnd ? -1 : 0);
 }
+
+// Tet types ordered by (b, c, a): tets sharing the pair (b, c) are consecutive, so
+// the cross product Xb x Xc is computed once per pair and dies right after.
+template <int D>
+struct TetOrder {
+  uint8_t t[64];
+};
+template <int D>
+constexpr TetOrder<D> build_tet_order() {
+  TetOrder<D> o{};
+  const auto C = build_chains<D, 3>();
+  bool used[64] = {};
+  for (int k = 0; k < C.n; ++k) {
+    int best = -1;
+    for (int t = 0; t < C.n; ++t) {
+      if (used[t]) continue;
+      if (best < 0) {
+        best = t;
+        continue;
+      }
+      const int kb = (C.s[best][2] << 8) | (C.s[best][3] << 4) | C.s[best][1];
+      const int kt = (C.s[t][2] << 8) | (C.s[t][3] << 4) | C.s[t][1];
+      if (kt < kb) best = t;
+    }
+    used[best] = true;
+    o.t[k] = (uint8_t)best;
+  }
+  return o;
+}
+template <int D>
+constexpr TetOrder<D> TET_ORDER = build_tet_order<D>();
 
 // ---------------------------------------------------------------------------
 // Per-vertex classification, two stages.
@@ -213,79 +272,122 @@ __device__ __forceinline__ int dot_sign(const double* e, const float* C, double 
 // a workgroup queue in LDS so every lane does useful work (eval_item).
 //
 // Bits: acc = accepted, sol = SoS inside, solver pending, deg = verbatim pending.
+// A triple product whose filtered FP64 value is within its error bound (exact zeros
+// included) sends the tet to the verbatim path, which decides it exactly.
+
+struct Masks {
+  uint64_t acc, sol, deg;
+};
+
+// LDS tile of quantized 3-component values.  A workgroup owns the 256 consecutive
+// vertices [v0, v0 + 256); the corners of all of them lie in 2^(D-1) runs of 257
+// consecutive vertices (one per combination of the non-x axes), each loaded
+// coalesced and quantized once.  Values are exact integers held in float (the
+// fast path requires |q| < 2^20).
+template <int D>
+struct Tile3 {
+  static constexpr int NSEG = 1 << (D - 1);
+  static constexpr int SEG = BLOCK + 1;
+  static constexpr int FLOATS = NSEG * SEG * 3;
+};
+
+template <class T, int D>
+__device__ __forceinline__ void stage_quantized3(const KParams& P, int64_t v0, float* tile) {
+  for (int e = threadIdx.x; e < Tile3<D>::NSEG * Tile3<D>::SEG; e += BLOCK) {
+    const int seg = e / Tile3<D>::SEG, i = e - seg * Tile3<D>::SEG;
+    int64_t u = v0 + P.loff[seg << 1] + i;
+    u = u < P.nvert ? u : P.nvert - 1;  // such corners lie outside the grid and are masked
+    VtxRef r;
+    if (P.packed) {
+      r.base[0] = r.base[1] = r.base[2] = u;
+    } else {
+      int64_t l[4];
+      decompose<D>(P, u, l);
+      r = make_ref<D>(P, u, l);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tile[e * 3 + k] = (float)qt(load_val<T>(P, k, r, 0), P.Q);
+  }
+}
 
 // Critical points of a 3-component field on 3-simplices (CriticalPointPredicate<3,T>,
 // predicate.hpp:66-99).
 template <class T, int D>
-__device__ __forceinline__ void classify_cp3(const KParams& P, const int64_t* l, uint64_t& acc,
-                                             uint64_t& sol, uint64_t& deg) {
+__device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, const float* tile) {
+  uint64_t acc = 0, sol = 0, deg = 0;
   constexpr int NC = 1 << D;
-  const uint32_t inb = corner_mask<D>(P, l);
+  const uint32_t inb = corner_mask<D>(P, x.l);
   float X[NC][3];
-  int64_t mx = 0;
+  double mx = 0;
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
+    const bool in = (inb >> s) & 1;
+    const float* src = tile + ((s >> 1) * Tile3<D>::SEG + threadIdx.x + (s & 1)) * 3;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      int64_t q = 0;
-      if ((inb >> s) & 1) q = q64(load_val<T, D>(P, k, l, s), P.Q);
-      const int64_t aq = q < 0 ? -q : q;
-      mx = aq > mx ? aq : mx;
-      X[s][k] = (float)q;
+      X[s][k] = in ? src[k] : 0.f;
+      mx = fmax(mx, (double)fabsf(X[s][k]));
     }
   }
-  acc = sol = deg = 0;
-  if (mx >= (int64_t(1) << 20)) {
+  if (!(mx < 0x1p20)) {
     // magnitudes beyond the fast filter: every tet takes the verbatim path
     sfor<0, CH<D, 3>.n>([&](auto tt) {
       constexpr int t = decltype(tt)::value;
-      if ((inb >> CH<D, 3>.s[t][3]) & 1) deg |= 1ull << t;
+      constexpr int c = CH<D, 3>.s[t][3];
+      if ((inb >> c) & 1) deg |= 1ull << t;
     });
-    return;
+    return Masks{acc, sol, deg};
   }
-  const double md = (double)mx;
-  const double bnd = md * md * md * 0x1p-48;
+  const double bnd = mx * mx * mx * 0x1p-48;
   uint64_t tpos = 0, tzero = 0;
   sfor<0, CH<D, 2>.n>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
-    if ((inb >> b) & 1) {
-      double e[3];
-      cross3(X[0], X[a], e);
-      const int sg = dot_sign(e, X[b], bnd);
-      if (sg > 0) tpos |= 1ull << t;
-      if (sg == 0) tzero |= 1ull << t;
-    }
+    // corners outside the grid hold 0 and are masked out by inb
+    double e[3];
+    cross3(X[0], X[a], e);
+    const int sg = dot_sign(e, X[b], bnd);
+    const uint64_t valid = (inb >> b) & 1;
+    tpos |= (valid & (uint64_t)(sg > 0)) << t;
+    tzero |= (valid & (uint64_t)(sg == 0)) << t;
+    __builtin_amdgcn_sched_barrier(0);  // keep live ranges short
   });
+  // signs of the opposite faces tp(a,b,c), in (b,c)-grouped order
+  uint64_t fpos = 0, func = 0;
+  sfor<0, CH<D, 3>.n>([&](auto kk) {
+    constexpr int t = TET_ORDER<D>.t[decltype(kk)::value];
+    constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
+    double e[3];
+    cross3(X[b], X[c], e);
+    const int sg = dot_sign(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
+    fpos |= (uint64_t)(sg > 0) << t;
+    func |= (uint64_t)(sg == 0) << t;
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  // d3 = det3(v0,v1,v2) = tp(0ab); d2 = det3(v0,v3,v1) = -tp(0ac);
+  // d1 = det3(v0,v2,v3) = tp(0bc); d0 = det3(v1,v3,v2) = -tp(abc).
+  // Inside by level 0 iff all four are equal; any uncertain/zero value -> verbatim.
   sfor<0, CH<D, 3>.n>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
     constexpr int i012 = tri_index<D>(a, b), i013 = tri_index<D>(a, c), i023 = tri_index<D>(b, c);
-    if ((inb >> c) & 1) {
-      const bool z = ((tzero >> i012) | (tzero >> i013) | (tzero >> i023)) & 1;
-      // d3 = det3(v0,v1,v2) = tp(0ab); d2 = det3(v0,v3,v1) = -tp(0ac);
-      // d1 = det3(v0,v2,v3) = tp(0bc); d0 = det3(v1,v3,v2) = -tp(abc)
-      const int p012 = (tpos >> i012) & 1, p013 = (tpos >> i013) & 1, p023 = (tpos >> i023) & 1;
-      if (z) {
-        deg |= 1ull << t;
-      } else if (p012 != p013 && p023 == p012) {
-        double e[3];
-        cross3(X[b], X[c], e);
-        const int sabc = dot_sign(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
-        if (sabc == 0)
-          deg |= 1ull << t;
-        else if (-sabc == (p012 ? 1 : -1))
-          sol |= 1ull << t;
-      }
-    }
+    const uint64_t valid = (inb >> c) & 1;
+    const uint64_t z = ((tzero >> i012) | (tzero >> i013) | (tzero >> i023) | (func >> t)) & 1;
+    const uint64_t p012 = (tpos >> i012) & 1, p013 = (tpos >> i013) & 1, p023 = (tpos >> i023) & 1;
+    const uint64_t pabc = (fpos >> t) & 1;
+    // d3 == d2 <=> p012 != p013;  d1 == d3 <=> p023 == p012;  d0 == d3 <=> pabc != p012
+    const uint64_t in = (p012 ^ p013) & ~(p023 ^ p012) & (pabc ^ p012) & 1;
+    deg |= (valid & z) << t;
+    sol |= (valid & ~z & in & 1) << t;
   });
+  return Masks{acc, sol, deg};
 }
 
 // det2-based predicates on 2-simplices: CriticalPointPredicate<2,T> and
 // FiberPredicate<T> (values minus thresholds, predicate.hpp:352).
 template <int KIND, class T, int D>
-__device__ __forceinline__ T val2(const KParams& P, int k, const int64_t* l, int s) {
-  const T v = load_val<T, D>(P, k, l, s);
+__device__ __forceinline__ T val2(const KParams& P, int k, const Vtx& x, int s) {
+  const T v = load_val<T>(P, k, x.r, s);
   if constexpr (KIND == KIND_FIBER) return v - (T)P.thr[k];
   return v;
 }
@@ -296,125 +398,134 @@ __device__ __forceinline__ double cross2(const double* A, const double* B) {
 }
 
 template <int KIND, class T, int D>
-__device__ __forceinline__ void classify_tri(const KParams& P, const int64_t* l, uint64_t& acc,
-                                             uint64_t& sol, uint64_t& deg) {
+__device__ __forceinline__ Masks classify_tri(const KParams& P, const Vtx& x) {
+  uint64_t acc = 0, sol = 0, deg = 0;
   constexpr int NC = 1 << D;
-  const uint32_t inb = corner_mask<D>(P, l);
+  const uint32_t inb = corner_mask<D>(P, x.l);
   double X[NC][2];
-  int64_t mx = 0;
+  double mx = 0;
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
+    const bool in = (inb >> s) & 1;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      int64_t q = 0;
-      if ((inb >> s) & 1) q = q64(val2<KIND, T, D>(P, k, l, s), P.Q);
-      const int64_t aq = q < 0 ? -q : q;
-      mx = aq > mx ? aq : mx;
-      X[s][k] = (double)q;
+      const T v = val2<KIND, T, D>(P, k, x, in ? s : 0);
+      const double q = in ? qt(v, P.Q) : 0.0;
+      mx = fmax(mx, fabs(q));
+      X[s][k] = q;
     }
   }
-  acc = sol = deg = 0;
-  const bool slow = mx >= (int64_t(1) << 26);
+  const bool slow = !(mx < 0x1p26);
   sfor<0, CH<D, 2>.n>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
-    if ((inb >> b) & 1) {
-      if (slow) {
-        deg |= 1ull << t;
-      } else {
-        const double c01 = cross2(X[0], X[a]);
-        const double c12 = cross2(X[a], X[b]);
-        const double c20 = cross2(X[b], X[0]);
-        const int s0 = c01 < 0 ? -1 : 1;  // tie: i0 < i1 -> +1
-        const int s1 = c12 < 0 ? -1 : 1;  // tie: i1 < i2 -> +1
-        const int s2 = c20 > 0 ? 1 : -1;  // tie: i2 > i0 -> -1
-        if (s0 == s1 && s1 == s2) {
-          if constexpr (KIND == KIND_CP)
-            sol |= 1ull << t;
-          else
-            acc |= 1ull << t;  // fiber: solver failure falls back to 1/3
-        }
-      }
-    }
+    const uint64_t valid = (inb >> b) & 1;
+    const double c01 = cross2(X[0], X[a]);
+    const double c12 = cross2(X[a], X[b]);
+    const double c20 = cross2(X[b], X[0]);
+    const bool s0 = c01 < 0;   // det2 tie: i0 < i1 -> +1
+    const bool s1 = c12 < 0;   // tie: i1 < i2 -> +1
+    const bool s2 = !(c20 > 0);  // tie: i2 > i0 -> -1
+    const uint64_t in = (uint64_t)(s0 == s1 && s1 == s2);
+    deg |= (valid & (uint64_t)slow) << t;
+    const uint64_t hit = valid & (uint64_t)!slow & in;
+    if constexpr (KIND == KIND_CP)
+      sol |= hit << t;
+    else
+      acc |= hit << t;  // fiber: solver failure falls back to 1/3
   });
+  return Masks{acc, sol, deg};
 }
 
 // Edge predicates: ContourPredicate<T> (sign change of value - threshold) and
 // CriticalPointPredicate<1,T> (sign change + solver gate).
 template <int KIND, class T, int D>
-__device__ __forceinline__ void classify_edge(const KParams& P, const int64_t* l, uint64_t& acc,
-                                              uint64_t& sol, uint64_t& deg) {
+__device__ __forceinline__ Masks classify_edge(const KParams& P, const Vtx& x) {
+  uint64_t acc = 0, sol = 0, deg = 0;
   constexpr int NC = 1 << D;
-  const uint32_t inb = corner_mask<D>(P, l);
+  const uint32_t inb = corner_mask<D>(P, x.l);
   int sg[NC];
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
-    sg[s] = 1;
-    if ((inb >> s) & 1) {
-      T v = load_val<T, D>(P, 0, l, s);
-      if constexpr (KIND == KIND_CONTOUR) v = v - (T)P.thr[0];
-      sg[s] = sos_sign(v, gid_of<D>(P, l, s), P.Q);
-    }
+    const bool in = (inb >> s) & 1;
+    T v = load_val<T>(P, 0, x.r, in ? s : 0);
+    if constexpr (KIND == KIND_CONTOUR) v = v - (T)P.thr[0];
+    sg[s] = in ? sos_sign(v, x.r.gid + P.goff[s], P.Q) : 1;
   }
-  acc = sol = deg = 0;
 #pragma unroll
   for (int s = 1; s < NC; ++s) {
-    if (((inb >> s) & 1) && sg[s] != sg[0]) {
-      // Chains<D,1> type of edge (0, s) is s - 1
-      if constexpr (KIND == KIND_CP)
-        sol |= 1ull << (s - 1);
-      else
-        acc |= 1ull << (s - 1);
-    }
+    // Chains<D,1> type of edge (0, s) is s - 1
+    const uint64_t hit = ((inb >> s) & 1) & (uint64_t)(sg[s] != sg[0]);
+    if constexpr (KIND == KIND_CP)
+      sol |= hit << (s - 1);
+    else
+      acc |= hit << (s - 1);
   }
+  return Masks{acc, sol, deg};
 }
 
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ void classify_fast(const KParams& P, const int64_t* l, uint64_t& acc,
-                                              uint64_t& sol, uint64_t& deg) {
+__device__ __forceinline__ Masks classify_fast(const KParams& P, const Vtx& x, const float* tile) {
   if constexpr (M == 3)
-    classify_cp3<T, D>(P, l, acc, sol, deg);
+    return classify_cp3<T, D>(P, x, tile);
   else if constexpr (M == 2)
-    classify_tri<KIND, T, D>(P, l, acc, sol, deg);
+    return classify_tri<KIND, T, D>(P, x);
   else
-    classify_edge<KIND, T, D>(P, l, acc, sol, deg);
+    return classify_edge<KIND, T, D>(P, x);
 }
 
 // Stage 2: one pending simplex (type t owned by the vertex at l).  deg: re-run the
 // verbatim sign tests; then the solver gate for critical points.
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ bool eval_item(const KParams& P, const int64_t* l, int t, bool deg) {
+__device__ __forceinline__ bool eval_item(const KParams& P, const VtxRef& r, int t, bool deg) {
   const uint32_t pc = packed_chain<D, M>(t);
   int off[M + 1];
   off[0] = 0;
 #pragma unroll
   for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
   if constexpr (M == 1) {
-    T v0 = load_val<T, D>(P, 0, l, off[0]);
-    T v1 = load_val<T, D>(P, 0, l, off[1]);
+    T v0 = load_val<T>(P, 0, r, off[0]);
+    T v1 = load_val<T>(P, 0, r, off[1]);
     T lam[2];
     return solve_1<T>(v0, v1, lam);  // only CriticalPointPredicate<1> queues edges
   } else {
     T vals[M + 1][M];
-    int64_t X[M + 1][M];
-    int64_t mx = 0;
 #pragma unroll
     for (int i = 0; i <= M; ++i)
 #pragma unroll
       for (int k = 0; k < M; ++k) {
-        T v = load_val<T, D>(P, k, l, off[i]);
+        T v = load_val<T>(P, k, r, off[i]);
         if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
         vals[i][k] = v;
-        X[i][k] = q64(v, P.Q);
-        const int64_t aq = X[i][k] < 0 ? -X[i][k] : X[i][k];
-        mx = aq > mx ? aq : mx;
       }
     if (deg) {
+      double Xd[M + 1][M];
+      double mx = 0;
+#pragma unroll
+      for (int i = 0; i <= M; ++i)
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          Xd[i][k] = qt(vals[i][k], P.Q);
+          mx = fmax(mx, fabs(Xd[i][k]));
+        }
       bool inside;
-      if constexpr (M == 3)
-        inside = mx < (int64_t(1) << 20) ? inside3_exact64(X) : inside3_exact(X);
-      else
+      if constexpr (M == 3) {
+        if (mx < 0x1p20) {
+          inside = inside3_exact_d(Xd, mx);
+        } else {
+          int64_t X[M + 1][M];
+          for (int i = 0; i <= M; ++i)
+            for (int k = 0; k < M; ++k) X[i][k] = q64(vals[i][k], P.Q);
+          inside = inside3_exact(X);
+        }
+      } else {
+        int64_t X[M + 1][M];
+#pragma unroll
+        for (int i = 0; i <= M; ++i)
+#pragma unroll
+          for (int k = 0; k < M; ++k) X[i][k] = q64(vals[i][k], P.Q);
         inside = inside2_exact(X);
+      }
       if (!inside) return false;
     }
     if constexpr (KIND == KIND_FIBER) return true;
@@ -426,7 +537,7 @@ __device__ __forceinline__ bool eval_item(const KParams& P, const int64_t* l, in
 // ---------------------------------------------------------------------------
 // lambda of one node (type t at vertex l), written as T[M+1]; returns scalar.
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ void node_lambda(const KParams& P, const int64_t* l, int t, T* lam,
+__device__ __forceinline__ void node_lambda(const KParams& P, const VtxRef& r, int t, T* lam,
                                             float* scalar) {
   const uint32_t pc = packed_chain<D, M>(t);
   int off[M + 1];
@@ -435,8 +546,8 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const int64_t* l, 
   for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
   *scalar = 0.f;
   if constexpr (M == 1) {
-    T v0 = load_val<T, D>(P, 0, l, off[0]);
-    T v1 = load_val<T, D>(P, 0, l, off[1]);
+    T v0 = load_val<T>(P, 0, r, off[0]);
+    T v1 = load_val<T>(P, 0, r, off[1]);
     if constexpr (KIND == KIND_CONTOUR) {
       v0 = v0 - (T)P.thr[0];
       v1 = v1 - (T)P.thr[0];
@@ -451,7 +562,7 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const int64_t* l, 
     for (int i = 0; i <= M; ++i)
 #pragma unroll
       for (int k = 0; k < M; ++k) {
-        vals[i][k] = load_val<T, D>(P, k, l, off[i]);
+        vals[i][k] = load_val<T>(P, k, r, off[i]);
         if constexpr (KIND == KIND_FIBER) vals[i][k] = vals[i][k] - (T)P.thr[k];
       }
     const bool ok = solve_k<M, T>(vals, lam);
@@ -465,7 +576,7 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const int64_t* l, 
     if (P.scal) {  // predicate.hpp:81-92
       T sv = T(0);
 #pragma unroll
-      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * load_scalar<T, D>(P, l, off[i]);
+      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * load_scalar<T>(P, r, off[i]);
       *scalar = (float)sv;
     }
   }
@@ -534,8 +645,7 @@ __device__ __forceinline__ void sort_small(uint32_t* c) {
 }
 
 template <int KIND, int D, int M>
-__device__ __forceinline__ uint32_t cells_of(const KParams& P, const int64_t* l, int64_t v, int mode,
-                                             uint64_t base) {
+__device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int mode, uint32_t* dst) {
   constexpr int CK = CellKind<KIND, D, M>::value;
   if constexpr (CK == 0) {
     return 0;
@@ -551,7 +661,7 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const int64_t* l,
       mk[o] = 0;
       no[o] = 0;
       if ((OWN >> o) & 1) {
-        const int64_t u = v + (o == 0 ? 0 : lin_of<D>(P, l, o) - v);
+        const int64_t u = x.v + P.loff[o];
         mk[o] = P.mask[u];
         if (mode) no[o] = P.node_off[u];
       }
@@ -563,9 +673,8 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const int64_t* l,
     auto put = [&](uint32_t* c) {
       if (mode) {
         sort_small<NV>(c);
-        uint32_t* dst = P.cells + (base + count) * NV;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) dst[i] = c[i];
+        for (int i = 0; i < NV; ++i) dst[count * NV + i] = c[i];
       }
       count++;
     };
@@ -763,20 +872,32 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const int64_t* l,
 // ---------------------------------------------------------------------------
 // Kernels
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK) void k_classify(KParams P) {
-  __shared__ uint16_t queue[BLOCK * 64];
+__global__ __launch_bounds__(BLOCK, 3) void k_classify(KParams P) {
+  constexpr int NTY = Chains<D, M>::N;
+  constexpr int QBYTES = BLOCK * NTY * 2;
+  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 0;
+  // stage 1 reads the quantized tile; stage 2 reuses the same bytes for its queue
+  __shared__ __align__(16) unsigned char smem[QBYTES > TBYTES ? QBYTES : TBYTES];
+  uint16_t* queue = reinterpret_cast<uint16_t*>(smem);
+  float* tile = reinterpret_cast<float*>(smem);
   __shared__ unsigned long long slot[BLOCK];
+  __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t qn;
   if (threadIdx.x == 0) qn = 0;
+  const int64_t v0 = (int64_t)blockIdx.x * BLOCK;
+  const int64_t v = v0 + threadIdx.x;
+  if constexpr (M == 3) stage_quantized3<T, D>(P, v0, tile);
   __syncthreads();
-  const int64_t base_v = (int64_t)blockIdx.x * BLOCK;
-  const int64_t v = base_v + threadIdx.x;
   uint64_t acc = 0, sol = 0, deg = 0;
   if (v < P.nvert) {
-    int64_t l[4];
-    decompose<D>(P, v, l);
-    classify_fast<KIND, D, M, T>(P, l, acc, sol, deg);
+    const Vtx x = make_vtx<D>(P, v);
+    refs[threadIdx.x] = x.r;
+    const Masks mk = classify_fast<KIND, D, M, T>(P, x, tile);
+    acc = mk.acc;
+    sol = mk.sol;
+    deg = mk.deg;
   }
+  if constexpr (M == 3) __syncthreads();  // the queue overwrites the tile
   slot[threadIdx.x] = acc;
   uint64_t w = sol | deg;
   if (w) {
@@ -793,9 +914,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(KParams P) {
     const uint32_t it = queue[i];
     const int owner = it & 0xFF, t = (it >> 8) & 0x3F;
     const bool dg = (it >> 14) & 1;
-    int64_t l[4];
-    decompose<D>(P, base_v + owner, l);
-    if (eval_item<KIND, D, M, T>(P, l, t, dg)) atomicOr(&slot[owner], 1ull << t);
+    if (eval_item<KIND, D, M, T>(P, refs[owner], t, dg)) atomicOr(&slot[owner], 1ull << t);
   }
   __syncthreads();
   const uint64_t m = slot[threadIdx.x];
@@ -830,96 +949,143 @@ static __global__ __launch_bounds__(1024) void k_scan_small(uint32_t* a, int64_t
   if (threadIdx.x == 1023) a[n] = part[1023];
 }
 
+template <int D>
+__device__ __forceinline__ bool is_hc_base(const KParams& P, const int64_t* l) {
+  bool hc = true;
+#pragma unroll
+  for (int j = 0; j < D; ++j) hc = hc && (l[j] + 1 < P.dims[j]);
+  return hc;
+}
+
 template <int KIND, int D, int M, class T>
 __global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
   const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   uint64_t m = 0;
-  int64_t l[4] = {0, 0, 0, 0};
+  uint32_t nc = 0;
   if (v < P.nvert) {
     m = P.mask[v];
-    decompose<D>(P, v, l);
+    Vtx x;
+    x.v = v;
+    decompose<D>(P, v, x.l);
+    if (is_hc_base<D>(P, x.l)) nc = cells_of<KIND, D, M>(P, x, 0, nullptr);
   }
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
   if (v < P.nvert) P.node_off[v] = P.bsum[blockIdx.x] + ex;
-  uint32_t nc = 0;
-  if (v < P.nvert) {
-    bool hc = true;
-#pragma unroll
-    for (int j = 0; j < D; ++j) hc = hc && (l[j] + 1 < P.dims[j]);
-    if (hc) nc = cells_of<KIND, D, M>(P, l, v, 0, 0);
-  }
   block_excl_scan(nc, &tot);
   if (threadIdx.x == 0) P.ebsum[blockIdx.x] = tot;
 }
 
+// Lexicographic insertion sort of n cells of NV indices (small n).
+template <int NV>
+__device__ __forceinline__ void sort_cells(uint32_t* seg, uint32_t n) {
+  for (uint32_t i = 1; i < n; ++i) {
+    uint32_t cur[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) cur[q] = seg[i * NV + q];
+    int j = (int)i - 1;
+    while (j >= 0) {
+      bool gt = false, eq = true;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const uint32_t a = seg[j * NV + q];
+        if (eq && a != cur[q]) {
+          gt = a > cur[q];
+          eq = false;
+        }
+      }
+      if (!gt) break;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) seg[(j + 1) * NV + q] = seg[j * NV + q];
+      j--;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) seg[(j + 1) * NV + q] = cur[q];
+  }
+}
+
+// Emit: the nodes of a workgroup occupy one contiguous output range in canonical
+// order (vertex, then type), so they are queued in LDS in that order and written by
+// consecutive threads (coalesced, load balanced).  Cells are staged in LDS, each
+// vertex's segment sorted, and copied out contiguously.
+constexpr int EMIT_NQ = 6144;          // queued nodes per workgroup
+constexpr int EMIT_STAGE_WORDS = 4096;  // staged cell indices per workgroup
+
 template <int KIND, int D, int M, class T>
 __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
   constexpr int NTY = Chains<D, M>::N;
+  constexpr int NV = CellKind<KIND, D, M>::verts;
+  __shared__ uint16_t nq[EMIT_NQ];
+  __shared__ VtxRef refs[BLOCK];
+  __shared__ uint32_t stage[EMIT_STAGE_WORDS];
   const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  int64_t l[4] = {0, 0, 0, 0};
+  Vtx x;
   uint64_t m = 0;
   bool hc = false;
   if (v < P.nvert) {
-    decompose<D>(P, v, l);
+    x = make_vtx<D>(P, v);
     m = P.mask[v];
-    hc = true;
-#pragma unroll
-    for (int j = 0; j < D; ++j) hc = hc && (l[j] + 1 < P.dims[j]);
+    hc = is_hc_base<D>(P, x.l);
+    refs[threadIdx.x] = x.r;
   }
-  // nodes
-  if (m) {
+  // ---- nodes
+  const uint32_t nbase = P.bsum[blockIdx.x];
+  const uint32_t ntot = P.bsum[blockIdx.x + 1] - nbase;
+  T* lam_out = static_cast<T*>(P.lambda);
+  auto emit_node = [&](uint32_t pos, const VtxRef& r, int t) {
+    T lam[M + 1];
+    float sc;
+    node_lambda<KIND, D, M, T>(P, r, t, lam, &sc);
+    P.keys[pos] = r.gid * (uint64_t)NTY + (uint64_t)t;
+#pragma unroll
+    for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
+    if (P.scalar_out) P.scalar_out[pos] = sc;
+    P.parent[pos] = pos;
+  };
+  if (ntot <= P.nq_cap) {
+    if (m) {
+      uint32_t pos = (v < P.nvert ? P.node_off[v] : 0) - nbase;
+      uint64_t mm = m;
+      while (mm) {
+        const int t = __builtin_ctzll(mm);
+        mm &= mm - 1;
+        nq[pos++] = (uint16_t)(threadIdx.x | (t << 8));
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) {
+      const uint32_t it = nq[i];
+      emit_node(nbase + i, refs[it & 0xFF], (int)(it >> 8));
+    }
+  } else if (m) {
     uint32_t pos = P.node_off[v];
-    const uint64_t g = gid_of<D>(P, l, 0);
-    T* lam_out = static_cast<T*>(P.lambda);
     uint64_t mm = m;
     while (mm) {
       const int t = __builtin_ctzll(mm);
       mm &= mm - 1;
-      T lam[M + 1];
-      float sc;
-      node_lambda<KIND, D, M, T>(P, l, t, lam, &sc);
-      P.keys[pos] = g * (uint64_t)NTY + (uint64_t)t;
-#pragma unroll
-      for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
-      if (P.scalar_out) P.scalar_out[pos] = sc;
-      P.parent[pos] = pos;
-      pos++;
+      emit_node(pos++, x.r, t);
     }
   }
-  // cells
-  uint32_t nc = 0;
-  if (hc) nc = cells_of<KIND, D, M>(P, l, v, 0, 0);
-  uint32_t tot;
-  const uint32_t ex = block_excl_scan(nc, &tot);
-  if (nc) {
-    const uint64_t base = (uint64_t)P.ebsum[blockIdx.x] + ex;
-    cells_of<KIND, D, M>(P, l, v, 1, base);
-    // sort this vertex's segment lexicographically (insertion sort, small)
-    constexpr int NV = CellKind<KIND, D, M>::verts;
-    uint32_t* seg = P.cells + base * NV;
-    for (uint32_t i = 1; i < nc; ++i) {
-      uint32_t cur[NV];
-#pragma unroll
-      for (int q = 0; q < NV; ++q) cur[q] = seg[i * NV + q];
-      int j = (int)i - 1;
-      while (j >= 0) {
-        bool gt = false, eq = true;
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-          const uint32_t a = seg[j * NV + q];
-          if (eq && a != cur[q]) {
-            gt = a > cur[q];
-            eq = false;
-          }
-        }
-        if (!gt) break;
-#pragma unroll
-        for (int q = 0; q < NV; ++q) seg[(j + 1) * NV + q] = seg[j * NV + q];
-        j--;
+  // ---- cells
+  if constexpr (NV > 0) {
+    uint32_t nc = 0;
+    if (hc) nc = cells_of<KIND, D, M>(P, x, 0, nullptr);
+    uint32_t ctot;
+    const uint32_t ex = block_excl_scan(nc, &ctot);
+    const uint64_t cbase = P.ebsum[blockIdx.x];
+    if (ctot * NV <= P.stage_cap) {
+      if (nc) {
+        uint32_t* seg = stage + ex * NV;
+        cells_of<KIND, D, M>(P, x, 1, seg);
+        sort_cells<NV>(seg, nc);
       }
-#pragma unroll
-      for (int q = 0; q < NV; ++q) seg[(j + 1) * NV + q] = cur[q];
+      __syncthreads();
+      uint32_t* dst = P.cells + cbase * NV;
+      for (uint32_t i = threadIdx.x; i < ctot * NV; i += BLOCK) dst[i] = stage[i];
+    } else if (nc) {
+      uint32_t* seg = P.cells + (cbase + ex) * NV;
+      cells_of<KIND, D, M>(P, x, 1, seg);
+      sort_cells<NV>(seg, nc);
     }
   }
 }

@@ -131,3 +131,13 @@ def test_run_twice_bit_identical():
     assert np.array_equal(a.lam.view(np.uint64), b.lam.view(np.uint64))
     assert np.array_equal(a.track_id, b.track_id)
     assert np.array_equal(a.connectivity[1], b.connectivity[1])
+
+
+def test_emit_overflow_paths(monkeypatch):
+    """Workgroups whose nodes or cells exceed the LDS staging capacity take the
+    per-thread path; results must not change."""
+    comps = synth.cp3d_config(n=22, nt=4, seed=12)
+    monkeypatch.setenv("FTK2H_EMIT_CAPS", "16,8")
+    compare("cp3d_emit_overflow", comps, "cp", 3)
+    s = synth.scalar_config(n=20, nt=4, ngauss=10, seed=2, dtype=np.float32)
+    compare("contour_emit_overflow", [s], "contour", 1)
