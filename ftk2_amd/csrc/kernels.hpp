@@ -699,13 +699,15 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
             sfor<0, NF>([&](auto kk) {
               constexpr int k = decltype(kk)::value;
               constexpr int f = CF<D, M>.order[p][k];
-              if (k == k0) hs = idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f]);
+              constexpr int o = CF<D, M>.owner[p][f], ty = CF<D, M>.type[p][f];
+              if (k == k0) hs = idx(o, ty);
             });
             sfor<0, NF>([&](auto kk) {
               constexpr int k = decltype(kk)::value;
               constexpr int f = CF<D, M>.order[p][k];
               if (((act >> k) & 1) && k != k0) {
-                uint32_t c[2] = {hs, idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f])};
+                constexpr int o = CF<D, M>.owner[p][f], ty = CF<D, M>.type[p][f];
+                uint32_t c[2] = {hs, idx(o, ty)};
                 put(c);
               }
             });
@@ -717,7 +719,8 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
             sfor<0, NF>([&](auto kk) {
               constexpr int k = decltype(kk)::value;
               constexpr int f = CF<D, M>.order[p][k];
-              if (k == k0) hs = idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f]);
+              constexpr int o = CF<D, M>.owner[p][f], ty = CF<D, M>.type[p][f];
+              if (k == k0) hs = idx(o, ty);
             });
           }
           sfor<0, D + 1>([&](auto ii) {
@@ -727,7 +730,8 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
             sfor<0, NF>([&](auto kk) {
               constexpr int k = decltype(kk)::value;
               constexpr int f = CF<D, M>.order[p][k];
-              if (!((CF<D, M>.vmask[f] >> i) & 1)) onf |= 1u << k;
+              constexpr int vm = CF<D, M>.vmask[f];
+              if (!((vm >> i) & 1)) onf |= 1u << k;
             });
             const uint32_t sel = act & onf;
             if (__builtin_popcount(sel) < 2) return;
@@ -743,13 +747,15 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
               sfor<0, NF>([&](auto kk) {
                 constexpr int k = decltype(kk)::value;
                 constexpr int f = CF<D, M>.order[p][k];
-                if (k == kt) ht = idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f]);
+                constexpr int o = CF<D, M>.owner[p][f], ty = CF<D, M>.type[p][f];
+                if (k == kt) ht = idx(o, ty);
               });
               sfor<0, NF>([&](auto kk) {
                 constexpr int k = decltype(kk)::value;
                 constexpr int f = CF<D, M>.order[p][k];
                 if ((rest >> k) & 1) {
-                  uint32_t c[3] = {hs, ht, idx(CF<D, M>.owner[p][f], CF<D, M>.type[p][f])};
+                  constexpr int o = CF<D, M>.owner[p][f], ty = CF<D, M>.type[p][f];
+                  uint32_t c[3] = {hs, ht, idx(o, ty)};
                   put(c);
                 }
               });
@@ -764,9 +770,13 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
       sfor<0, NPC>([&](auto pp) {
         constexpr int p = decltype(pp)::value;
         int rel[D + 1];
-        rel[0] = 0;
-#pragma unroll
-        for (int i = 1; i <= D; ++i) rel[i] = (int)((mk[0] >> (CF<D, 1>.cell[p][i] - 1)) & 1);
+        int ce[D + 1];  // cell vertex offsets (compile-time constants)
+        sfor<0, D + 1>([&](auto qq) {
+          constexpr int q = decltype(qq)::value;
+          constexpr int cq = CF<D, 1>.cell[p][q];
+          ce[q] = cq;
+          rel[q] = q == 0 ? 0 : (int)((mk[0] >> (cq > 0 ? cq - 1 : 0)) & 1);
+        });
         int g1 = 0;
 #pragma unroll
         for (int i = 0; i <= D; ++i) g1 += rel[i];
@@ -776,8 +786,8 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
           int ci = 0, cj = 0;
 #pragma unroll
           for (int q = 0; q <= D; ++q) {
-            if (q == i) ci = CF<D, 1>.cell[p][q];
-            if (q == j) cj = CF<D, 1>.cell[p][q];
+            if (q == i) ci = ce[q];
+            if (q == j) cj = ce[q];
           }
           const int lo = i < j ? ci : cj, hi = i < j ? cj : ci;
           uint32_t r = 0;
