@@ -124,6 +124,8 @@ void ftk2h_ctx_destroy(ftk2h_ctx* ctx) {
   ctx->node_off.release();
   ctx->bsum.release();
   ctx->ebsum.release();
+  ctx->xedges.release();
+  ctx->xcount.release();
   for (auto& f : ctx->fields) f.release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -298,6 +300,8 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   P.node_off = static_cast<uint32_t*>(ctx->node_off.p);
   P.bsum = static_cast<uint32_t*>(ctx->bsum.p);
   P.ebsum = static_cast<uint32_t*>(ctx->ebsum.p);
+  if ((rc = ctx->xcount.ensure(256))) return rc;
+  P.xcount = static_cast<uint32_t*>(ctx->xcount.p);
 
   RunArgs A{ctx, st, P, nb, opt ? opt->node_capacity : 0, opt ? opt->cell_capacity : 0, true, res.get()};
   rc = fn(A);

@@ -65,7 +65,7 @@ constexpr int NPHASE = 7;
 struct ftk2h_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  ftk2h_internal::DevBuf mask, node_off, bsum, ebsum, fields[4];
+  ftk2h_internal::DevBuf mask, node_off, bsum, ebsum, xedges, xcount, fields[4];
   hipEvent_t ev[ftk2h_internal::NPHASE + 1] = {};
   std::mutex lock;
 };
@@ -157,6 +157,9 @@ int run_pipeline(RunArgs& A) {
     if (P.scal) HIP_TRY(hipMalloc(&R->scalar, nn * 4));
   }
   if (NV > 0 && nc) {
+    int rc = ctx->xedges.ensure(nc * (uint64_t)(NV - 1) * 8);
+    if (rc) return rc;
+    P.xedges = static_cast<uint32_t*>(ctx->xedges.p);
     R->n_conn = 1;
     R->conn_dim[0] = NV - 1;
     R->conn_cells[0] = nc;
@@ -175,8 +178,9 @@ int run_pipeline(RunArgs& A) {
   HIP_TRY(rec(5));
   if constexpr (NV > 0) {
     if (nc) {
-      const uint64_t g = (nc + BLOCK - 1) / BLOCK;
-      k_link<NV><<<dim3((unsigned)g), dim3(BLOCK), 0, st>>>(parent, P.cells, nc);
+      // at most nc * (NV - 1) pairs; threads beyond the device-side count exit
+      const uint64_t g = (nc * (NV - 1) + BLOCK - 1) / BLOCK;
+      k_link_pairs<<<dim3((unsigned)g), dim3(BLOCK), 0, st>>>(parent, P.xedges, P.xcount);
       HIP_TRY(hipGetLastError());
     }
   }

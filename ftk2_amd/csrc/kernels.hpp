@@ -62,6 +62,8 @@ struct KParams {
   void* lambda;
   float* scalar_out;
   uint32_t* cells;
+  uint32_t* xedges;  // edges still to be united globally: (a, b) pairs
+  uint32_t* xcount;  // number of pairs in xedges
   uint32_t* parent;
   uint32_t* track;
   uint64_t n_nodes;
@@ -982,6 +984,7 @@ __global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
   if (v < P.nvert) P.node_off[v] = P.bsum[blockIdx.x] + ex;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *P.xcount = 0;
   block_excl_scan(nc, &tot);
   if (threadIdx.x == 0) P.ebsum[blockIdx.x] = tot;
 }
@@ -1019,6 +1022,35 @@ __device__ __forceinline__ void sort_cells(uint32_t* seg, uint32_t n) {
 // consecutive threads (coalesced, load balanced).  Cells are staged in LDS, each
 // vertex's segment sorted, and copied out contiguously.
 constexpr int EMIT_NQ = 6144;          // queued nodes per workgroup
+constexpr int EMIT_LUF = EMIT_NQ / 2;   // nodes of the workgroup union-find (u32 in nq)
+
+// Union-find in LDS over one workgroup's nodes (hook the larger root under the
+// smaller, so roots are minima).
+__device__ __forceinline__ uint32_t lds_find(uint32_t* p, uint32_t x) {
+  while (true) {
+    const uint32_t y = __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (y == x) return x;
+    const uint32_t z = __hip_atomic_load(p + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (z != y) __hip_atomic_store(p + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = y;
+  }
+}
+__device__ __forceinline__ void lds_unite(uint32_t* p, uint32_t a, uint32_t b) {
+  while (true) {
+    a = lds_find(p, a);
+    b = lds_find(p, b);
+    if (a == b) return;
+    if (a > b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t expected = b;
+    if (__hip_atomic_compare_exchange_strong(p + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+  }
+}
 constexpr int EMIT_STAGE_WORDS = 4096;  // staged cell indices per workgroup
 
 template <int KIND, int D, int M, class T>
@@ -1050,7 +1082,6 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
 #pragma unroll
     for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
     if (P.scalar_out) P.scalar_out[pos] = sc;
-    P.parent[pos] = pos;
   };
   if (ntot <= P.nq_cap) {
     if (m) {
@@ -1076,6 +1107,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
       emit_node(pos++, x.r, t);
     }
   }
+  bool linked = false;
   // ---- cells
   if constexpr (NV > 0) {
     uint32_t nc = 0;
@@ -1092,12 +1124,73 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
       __syncthreads();
       uint32_t* dst = P.cells + cbase * NV;
       for (uint32_t i = threadIdx.x; i < ctot * NV; i += BLOCK) dst[i] = stage[i];
+      // Union the cells whose nodes all belong to this workgroup in LDS and start
+      // the global union-find from those roots (roots stay component minima).
+      if (ntot <= EMIT_LUF) {
+        uint32_t* lpar = reinterpret_cast<uint32_t*>(nq);  // the node queue is done
+        for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) lpar[i] = i;
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
+          const uint32_t a = stage[c * NV] - nbase;
+          if (a >= ntot) continue;
+#pragma unroll
+          for (int q = 1; q < NV; ++q) {
+            const uint32_t b = stage[c * NV + q] - nbase;
+            if (b < ntot) lds_unite(lpar, a, b);
+          }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) P.parent[nbase + i] = nbase + lds_find(lpar, i);
+        // edges leaving the workgroup, from the local root of their first node
+        uint32_t nx = 0;
+        for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
+          const uint32_t a = stage[c * NV] - nbase;
+#pragma unroll
+          for (int q = 1; q < NV; ++q) nx += (a >= ntot || stage[c * NV + q] - nbase >= ntot) ? 1u : 0u;
+        }
+        uint32_t xtot;
+        uint32_t xo = block_excl_scan(nx, &xtot);
+        __shared__ uint32_t xbase;
+        if (threadIdx.x == 0) xbase = xtot ? atomicAdd(P.xcount, xtot) : 0;
+        __syncthreads();
+        xo += xbase;
+        for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
+          const uint32_t a = stage[c * NV] - nbase;
+          const uint32_t ra = a < ntot ? nbase + lds_find(lpar, a) : stage[c * NV];
+#pragma unroll
+          for (int q = 1; q < NV; ++q) {
+            const uint32_t b = stage[c * NV + q];
+            if (a >= ntot || b - nbase >= ntot) {
+              P.xedges[2 * (uint64_t)xo] = ra;
+              P.xedges[2 * (uint64_t)xo + 1] = b;
+              xo++;
+            }
+          }
+        }
+        linked = true;
+      }
     } else if (nc) {
       uint32_t* seg = P.cells + (cbase + ex) * NV;
       cells_of<KIND, D, M>(P, x, 1, seg);
       sort_cells<NV>(seg, nc);
     }
+    if (!linked) {
+      // every edge of this workgroup goes to the global union-find
+      __syncthreads();  // cells written to global memory by their threads
+      const uint32_t ne = ctot * (NV - 1);
+      __shared__ uint32_t xbase2;
+      if (threadIdx.x == 0) xbase2 = ne ? atomicAdd(P.xcount, ne) : 0;
+      __syncthreads();
+      const uint32_t* src = P.cells + cbase * NV;
+      for (uint32_t e = threadIdx.x; e < ne; e += BLOCK) {
+        const uint32_t c = e / (NV - 1), q = 1 + e % (NV - 1);
+        P.xedges[2 * (uint64_t)(xbase2 + e)] = src[c * NV];
+        P.xedges[2 * (uint64_t)(xbase2 + e) + 1] = src[c * NV + q];
+      }
+    }
   }
+  if (!linked)
+    for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) P.parent[nbase + i] = nbase + i;
 }
 
 // union-find hooking (larger root under smaller) over every cell
@@ -1132,14 +1225,11 @@ __device__ __forceinline__ void uf_unite(uint32_t* parent, uint32_t a, uint32_t 
   }
 }
 
-template <int NV>
-__global__ __launch_bounds__(BLOCK) void k_link(uint32_t* parent, const uint32_t* cells, uint64_t n) {
+static __global__ __launch_bounds__(BLOCK) void k_link_pairs(uint32_t* parent, const uint32_t* pairs,
+                                                      const uint32_t* count) {
   const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t* c = cells + i * NV;
-  const uint32_t a = c[0];
-#pragma unroll
-  for (int q = 1; q < NV; ++q) uf_unite(parent, a, c[q]);
+  if (i >= *count) return;
+  uf_unite(parent, pairs[2 * i], pairs[2 * i + 1]);
 }
 
 static __global__ __launch_bounds__(BLOCK) void k_compress(uint32_t* parent, uint32_t* track, uint64_t n) {
