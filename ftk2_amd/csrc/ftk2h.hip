@@ -26,6 +26,10 @@ extern template int run_pipeline<KIND_CONTOUR, 3, 1, float>(RunArgs&);
 extern template int run_pipeline<KIND_CONTOUR, 3, 1, double>(RunArgs&);
 extern template int run_pipeline<KIND_CONTOUR, 2, 1, float>(RunArgs&);
 extern template int run_pipeline<KIND_CONTOUR, 2, 1, double>(RunArgs&);
+extern template int run_pipeline<KIND_FIBER, 4, 2, float>(RunArgs&);
+extern template int run_pipeline<KIND_FIBER, 4, 2, double>(RunArgs&);
+extern template int run_pipeline<KIND_FIBER, 3, 2, float>(RunArgs&);
+extern template int run_pipeline<KIND_FIBER, 3, 2, double>(RunArgs&);
 
 struct Entry {
   int kind, d, m, dtype;
@@ -51,6 +55,9 @@ const Entry kEntries[] = {
     entry_d<KIND_CONTOUR, 4, 1>(), entry_f<KIND_CONTOUR, 4, 1>(),
     entry_d<KIND_CONTOUR, 3, 1>(), entry_f<KIND_CONTOUR, 3, 1>(),
     entry_d<KIND_CONTOUR, 2, 1>(), entry_f<KIND_CONTOUR, 2, 1>(),
+    // fiber
+    entry_d<KIND_FIBER, 4, 2>(), entry_f<KIND_FIBER, 4, 2>(),
+    entry_d<KIND_FIBER, 3, 2>(), entry_f<KIND_FIBER, 3, 2>(),
 };
 
 RunFn find_entry(int kind, int d, int m, int dtype) {

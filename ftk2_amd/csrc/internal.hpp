@@ -13,6 +13,9 @@
 
 #include "../../include/ftk2h.h"
 #include "kernels.hpp"
+#include "sortcells.hpp"
+
+#include <algorithm>
 
 namespace ftk2h_internal {
 using namespace ftk2h;
@@ -113,6 +116,58 @@ struct RunArgs {
 
 using RunFn = int (*)(RunArgs&);
 
+// Sort + unique the cells of R->conn[0] (bucket by smallest index, sort inside
+// buckets, compact).  Replaces R->conn[0] and R->conn_cells[0].
+template <int NV>
+int sort_unique_cells(ftk2h_result* R, uint64_t nn, uint64_t nc, hipStream_t st) {
+  uint32_t *cnt = nullptr, *sums = nullptr, *tmp = nullptr, *keep = nullptr, *pos = nullptr, *out = nullptr;
+  const uint64_t tiles = (std::max(nn, nc) + 1 + SCAN_TILE - 1) / SCAN_TILE + 2;
+  auto cleanup = [&]() {
+    for (uint32_t* p : {cnt, sums, tmp, keep, pos})
+      if (p) (void)hipFree(p);
+  };
+  auto check = [&](hipError_t e, const char* what) -> int {
+    if (e == hipSuccess) return FTK2H_OK;
+    cleanup();
+    return fail(e == hipErrorOutOfMemory ? FTK2H_E_OOM : FTK2H_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  int rc;
+  if ((rc = check(hipMalloc(&cnt, (nn + 1) * 4), "hipMalloc"))) return rc;
+  if ((rc = check(hipMalloc(&sums, tiles * 4), "hipMalloc"))) return rc;
+  if ((rc = check(hipMalloc(&tmp, nc * NV * 4), "hipMalloc"))) return rc;
+  if ((rc = check(hipMalloc(&keep, (nc + 1) * 4), "hipMalloc"))) return rc;
+  const unsigned gn = (unsigned)((nn + BLOCK) / BLOCK), gc = (unsigned)((nc + BLOCK - 1) / BLOCK);
+  k_fill_u32<<<gn, BLOCK, 0, st>>>(cnt, 0u, nn + 1);
+  k_cell_count<NV><<<gc, BLOCK, 0, st>>>(R->conn[0], nc, cnt);
+  if ((rc = check(scan_u32(cnt, cnt, nn, sums, st), "scan"))) return rc;  // cnt -> bucket offsets
+  // scatter with a cursor copy of the offsets (keep holds the cursor temporarily)
+  uint32_t* cursor = keep;
+  if (nc + 1 < nn + 1) {
+    (void)hipFree(keep);
+    keep = nullptr;
+    if ((rc = check(hipMalloc(&keep, (nn + 1) * 4), "hipMalloc"))) return rc;
+    cursor = keep;
+  }
+  if ((rc = check(hipMemcpyAsync(cursor, cnt, (nn + 1) * 4, hipMemcpyDeviceToDevice, st), "copy"))) return rc;
+  k_cell_scatter<NV><<<gc, BLOCK, 0, st>>>(R->conn[0], nc, cursor, tmp);
+  k_fill_u32<<<(unsigned)((nc + BLOCK) / BLOCK), BLOCK, 0, st>>>(keep, 0u, nc + 1);
+  k_cell_bucket_sort<NV><<<gn, BLOCK, 0, st>>>(tmp, cnt, nn, keep);
+  // positions of kept cells
+  if ((rc = check(hipMalloc(&pos, (nc + 1) * 4), "hipMalloc"))) return rc;
+  if ((rc = check(scan_u32(keep, pos, nc, sums, st), "scan"))) return rc;
+  uint32_t nu = 0;
+  if ((rc = check(hipMemcpyAsync(&nu, pos + nc, 4, hipMemcpyDeviceToHost, st), "copy"))) return rc;
+  if ((rc = check(hipStreamSynchronize(st), "sync"))) return rc;
+  if ((rc = check(hipMalloc(&out, (uint64_t)std::max<uint32_t>(nu, 1) * NV * 4), "hipMalloc"))) return rc;
+  k_cell_compact<NV><<<gc, BLOCK, 0, st>>>(tmp, keep, pos, nc, out);
+  if ((rc = check(hipStreamSynchronize(st), "sync"))) return rc;
+  (void)hipFree(R->conn[0]);
+  R->conn[0] = out;
+  R->conn_cells[0] = nu;
+  cleanup();
+  return FTK2H_OK;
+}
+
 template <int KIND, int D, int M, class T>
 int run_pipeline(RunArgs& A) {
   constexpr int NV = CellKind<KIND, D, M>::verts;
@@ -191,6 +246,13 @@ int run_pipeline(RunArgs& A) {
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(rec(7));
+  if constexpr (CellKind<KIND, D, M>::value == 2) {
+    // star-fan triangles: global sort + unique (FeatureComplex::sort)
+    if (nc) {
+      int rc = sort_unique_cells<NV>(R, nn, nc, st);
+      if (rc) return rc;
+    }
+  }
   // parent is scratch; free after the stream drains
   HIP_TRY(hipStreamSynchronize(st));
   if (parent) HIP_TRY(hipFree(parent));

@@ -1232,6 +1232,11 @@ static __global__ __launch_bounds__(BLOCK) void k_link_pairs(uint32_t* parent, c
   uf_unite(parent, pairs[2 * i], pairs[2 * i + 1]);
 }
 
+static __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 static __global__ __launch_bounds__(BLOCK) void k_compress(uint32_t* parent, uint32_t* track, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;

@@ -141,3 +141,25 @@ def test_emit_overflow_paths(monkeypatch):
     compare("cp3d_emit_overflow", comps, "cp", 3)
     s = synth.scalar_config(n=20, nt=4, ngauss=10, seed=2, dtype=np.float32)
     compare("contour_emit_overflow", [s], "contour", 1)
+
+
+def test_fiber3d_reference_case():
+    cx, _ = compare("fiber3d_double_32^3x10", refcases.fiber3d_double(), "fiber", 2)
+    assert len(cx.vertices) == 6078
+
+
+def test_fiber_float_config_like():
+    a = synth.scalar_config(n=20, nt=4, ngauss=10, seed=5, dtype=np.float32)
+    b = synth.scalar_config(n=20, nt=4, ngauss=10, seed=6, dtype=np.float32)
+    compare("fiber_float_20^3x4", [a, b], "fiber", 2)
+
+
+def test_fiber_on_3d_mesh_star_edges():
+    a = synth.gaussians((26, 24), 5, 6, gradient=False, seed=8, subtract_mean=True)[0]
+    b = synth.gaussians((26, 24), 5, 6, gradient=False, seed=9, subtract_mean=True)[0]
+    compare("fiber_2d+t", [a, b], "fiber", 2, thr=(0.001, -0.002))
+
+
+def test_cp2_on_4d_mesh_star_fan():
+    u, v = synth.gaussians((12, 11, 10), 3, 5, sigma=(3, 6), seed=10)[:2]
+    compare("cp2_on_3d+t", [u, v], "cp", 2)
