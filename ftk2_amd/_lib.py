@@ -65,6 +65,11 @@ SIGNATURES = {
     "ftk2h_result_phase_ms": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
     "ftk2h_result_free": (None, [_P]),
     "ftk2h_last_error": (C.c_char_p, []),
+    "ftk2h_stream_open": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(PredicateDesc), C.c_int,
+                                    C.POINTER(_P)]),
+    "ftk2h_stream_append": (C.c_int, [_P, C.POINTER(FieldView), _P]),
+    "ftk2h_stream_finish": (C.c_int, [_P, C.POINTER(_P)]),
+    "ftk2h_stream_close": (None, [_P]),
 }
 
 _lib = None

@@ -66,12 +66,26 @@ def build_synth() -> str:
     return LIB_SYNTH
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_examples_gpu.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_examples_gpu")
+
+
+def build_cpp_tests() -> str:
+    """The C++ program written against the drop-in headers (include/ftk2/...)."""
+    tmp = CPP_TEST_BIN + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
+          "-L", HERE, "-lftk2h", "-Wl,-rpath,$ORIGIN/../../ftk2_amd", "-o", tmp])
+    os.replace(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build_all() -> None:
     build_synth()
     build_hip()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "synth": build_synth}[w]()
+        {"all": build_all, "hip": build_hip, "synth": build_synth, "cpp": build_cpp_tests}[w]()

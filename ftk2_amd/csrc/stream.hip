@@ -1,0 +1,395 @@
+// Streaming extraction (SimplicialEngine::execute_stream, hguo/ftk2
+// include/ftk2/core/engine.hpp:214-270): timesteps arrive one at a time; each new
+// timestep t closes the slab [t-1, t], a RegularSimplicialMesh with local extent 2 in
+// time and offset t-1 (engine.hpp:261-266).  Device-resident state:
+//   * the last timestep of every field component (ping-pong halves of one buffer,
+//     so a slab is a plain strided view);
+//   * the previous slab's result: its upper-plane nodes (pure spatial simplices at
+//     time t) are also nodes of the next slab, so that slab stays open until the
+//     next one arrives and its edges can be re-indexed by key;
+//   * the committed nodes/edges and a union-find over them (the open frontier of
+//     the track components).
+// The result of finish() equals the one-shot extraction over all timesteps (the
+// reference's contract, test_examples.cpp:259-297), which the tests check.
+#include "internal.hpp"
+
+using namespace ftk2h;
+using namespace ftk2h_internal;
+
+namespace {
+
+// Grow-to-fit device array that keeps its contents.
+struct DevVec {
+  void* p = nullptr;
+  size_t bytes = 0, cap = 0;
+  int reserve(size_t n, hipStream_t st) {
+    if (n <= cap) return FTK2H_OK;
+    size_t nc = cap ? cap : (size_t)1 << 20;
+    while (nc < n) nc *= 2;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, nc);
+    if (e != hipSuccess) return fail(FTK2H_E_OOM, "stream buffer: " + std::string(hipGetErrorString(e)));
+    if (bytes) {
+      e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return fail(FTK2H_E_HIP, std::string("stream copy: ") + hipGetErrorString(e));
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+    cap = nc;
+    return FTK2H_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = cap = 0;
+  }
+};
+
+// number of keys below `bound` (keys sorted ascending)
+__global__ void k_lower_bound(const uint64_t* keys, uint64_t n, uint64_t bound, uint32_t* out) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (keys[mid] < bound)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  *out = (uint32_t)lo;
+}
+
+// Re-index the cells of a slab result to committed indices: lower-plane nodes are
+// committed at base + i; upper-plane nodes are found by key among the next slab's
+// lower-plane nodes (committed at next_base + position).  Each cell is then united.
+template <int NV>
+__global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, uint64_t nc, const uint64_t* keys,
+                                                       uint32_t nlow, uint32_t base, const uint64_t* next_keys,
+                                                       uint32_t next_nlow, uint32_t next_base, uint32_t* out,
+                                                       uint32_t* parent, uint32_t* bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nc) return;
+  uint32_t c[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const uint32_t x = cells[i * NV + q];
+    if (x < nlow) {
+      c[q] = base + x;
+    } else {
+      const uint64_t key = keys[x];
+      uint32_t lo = 0, hi = next_nlow;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (next_keys[mid] < key)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo >= next_nlow || next_keys[lo] != key) {
+        atomicAdd(bad, 1u);
+        lo = 0;
+      }
+      c[q] = next_base + lo;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[i * NV + q] = c[q];
+#pragma unroll
+  for (int q = 1; q < NV; ++q) uf_unite(parent, c[0], c[q]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_iota(uint32_t* p, uint32_t base, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) p[i] = base + (uint32_t)i;
+}
+
+}  // namespace
+
+struct ftk2h_stream {
+  ftk2h_ctx* ctx = nullptr;
+  int sd = 0;
+  uint64_t spatial[3] = {1, 1, 1};
+  uint64_t nsp = 0;
+  ftk2h_predicate_desc pred{};
+  int ncomp = 0;
+  size_t esz = 8;
+  int ntypes = 0;
+  int nv = 0;  // indices per cell (0: no connectivity)
+  hipStream_t st = nullptr;
+  void* fields[3] = {nullptr, nullptr, nullptr};  // 2 * nsp values each (ping-pong)
+  uint64_t t = 0;                                 // timesteps appended
+  ftk2h_result* open = nullptr;                   // previous slab, not committed yet
+  uint32_t open_nlow = 0;
+  uint32_t committed = 0;                         // committed nodes
+  uint64_t committed_cells = 0;
+  DevVec keys, lambda, parent, cells;
+  uint32_t* scratch = nullptr;  // small device words (lower bound, bad-key count)
+  float ms_total = 0.f;
+};
+
+namespace {
+
+int slab_lower_count(ftk2h_stream* S, const ftk2h_result* R, uint64_t slab_t, uint32_t* out) {
+  const uint64_t bound = (slab_t + 1) * S->nsp * (uint64_t)S->ntypes;
+  k_lower_bound<<<1, 1, 0, S->st>>>(R->keys, R->n_nodes, bound, S->scratch);
+  HIP_TRY(hipMemcpyAsync(out, S->scratch, 4, hipMemcpyDeviceToHost, S->st));
+  HIP_TRY(hipStreamSynchronize(S->st));
+  return FTK2H_OK;
+}
+
+// Commit the open slab: its first `nlow` nodes (or all of them when `next` is null)
+// and its cells re-indexed against `next`.
+int commit_open(ftk2h_stream* S, const ftk2h_result* next, uint32_t next_nlow) {
+  ftk2h_result* R = S->open;
+  const uint32_t nk = next ? S->open_nlow : (uint32_t)R->n_nodes;
+  const size_t lsz = (size_t)(R->m + 1) * S->esz;
+  int rc;
+  const uint64_t n_after = (uint64_t)S->committed + nk + (next ? next_nlow : 0);
+  if ((rc = S->keys.reserve(n_after * 8, S->st))) return rc;
+  if ((rc = S->lambda.reserve(n_after * lsz, S->st))) return rc;
+  if ((rc = S->parent.reserve(n_after * 4, S->st))) return rc;
+  char* kb = static_cast<char*>(S->keys.p);
+  char* lb = static_cast<char*>(S->lambda.p);
+  if (nk) {
+    HIP_TRY(hipMemcpyAsync(kb + (size_t)S->committed * 8, R->keys, (size_t)nk * 8, hipMemcpyDeviceToDevice, S->st));
+    HIP_TRY(hipMemcpyAsync(lb + (size_t)S->committed * lsz, R->lambda, (size_t)nk * lsz, hipMemcpyDeviceToDevice,
+                           S->st));
+  }
+  // parents of this slab's nodes and of the next slab's lower plane (cells point there)
+  const uint64_t fresh = nk + (next ? next_nlow : 0);
+  uint32_t* par = static_cast<uint32_t*>(S->parent.p);
+  const uint64_t already = S->parent.bytes / 4;  // parents initialised so far
+  if ((uint64_t)S->committed + fresh > already) {
+    const uint64_t from = already, cnt = (uint64_t)S->committed + fresh - already;
+    k_iota<<<(unsigned)((cnt + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(par + from, (uint32_t)from, cnt);
+    S->parent.bytes = ((uint64_t)S->committed + fresh) * 4;
+  }
+  if (S->nv && R->n_conn) {
+    const uint64_t nc = R->conn_cells[0];
+    if ((rc = S->cells.reserve((S->committed_cells + nc) * S->nv * 4, S->st))) return rc;
+    uint32_t* dst = static_cast<uint32_t*>(S->cells.p) + S->committed_cells * S->nv;
+    HIP_TRY(hipMemsetAsync(S->scratch + 1, 0, 4, S->st));
+    const unsigned g = (unsigned)((nc + BLOCK - 1) / BLOCK);
+    const uint64_t* nkeys = next ? next->keys : nullptr;
+    const uint32_t nb = S->committed + nk;
+    switch (S->nv) {
+      case 2: k_remap_cells<2><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
+      case 3: k_remap_cells<3><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
+      case 4: k_remap_cells<4><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
+    }
+    HIP_TRY(hipGetLastError());
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, S->scratch + 1, 4, hipMemcpyDeviceToHost, S->st));
+    HIP_TRY(hipStreamSynchronize(S->st));
+    if (bad) return fail(FTK2H_E_HIP, "stream: upper-plane node missing from the next slab");
+    S->committed_cells += nc;
+    S->cells.bytes = S->committed_cells * S->nv * 4;
+  }
+  S->committed += nk;
+  S->keys.bytes = (size_t)S->committed * 8;
+  S->lambda.bytes = (size_t)S->committed * lsz;
+  return FTK2H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial_dims,
+                      const ftk2h_predicate_desc* pred, int ncomp, ftk2h_stream** out) {
+  if (!ctx || !spatial_dims || !pred || !out) return fail(FTK2H_E_INVALID, "null argument");
+  if (spatial_ndims < 1 || spatial_ndims > 3) return fail(FTK2H_E_INVALID, "spatial dims must be 1..3");
+  if (ncomp < 1 || ncomp > 3) return fail(FTK2H_E_INVALID, "ncomp must be 1..3");
+  auto S = std::make_unique<ftk2h_stream>();
+  S->ctx = ctx;
+  S->sd = spatial_ndims;
+  S->nsp = 1;
+  for (int j = 0; j < spatial_ndims; ++j) {
+    if (spatial_dims[j] == 0) return fail(FTK2H_E_INVALID, "zero extent");
+    S->spatial[j] = spatial_dims[j];
+    S->nsp *= spatial_dims[j];
+  }
+  S->pred = *pred;
+  S->ncomp = ncomp;
+  S->esz = pred->dtype == FTK2H_F64 ? 8 : 4;
+  const int d = spatial_ndims + 1, m = pred->codimension;
+  S->ntypes = n_types_rt(d, m);
+  if (!S->ntypes) return fail(FTK2H_E_UNSUPPORTED, "no simplices of this dimension");
+  // cells: star edges (k=1), star-fan (k=2), marching tets/triangles (contour)
+  const int k = d - m;
+  if (pred->kind == FTK2H_PRED_CONTOUR && d == 4) S->nv = 4;
+  else if (pred->kind == FTK2H_PRED_CONTOUR && d == 3) S->nv = 3;
+  else if (m == 1 && d >= 3) S->nv = 0;
+  else S->nv = k == 1 ? 2 : (k == 2 ? 3 : 0);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+  for (int c = 0; c < ncomp; ++c) HIP_TRY(hipMalloc(&S->fields[c], 2 * S->nsp * S->esz));
+  HIP_TRY(hipMalloc(&S->scratch, 64));
+  *out = S.release();
+  return FTK2H_OK;
+}
+
+int ftk2h_stream_append(ftk2h_stream* S, const ftk2h_field_view* comps, void* hip_stream) {
+  if (!S || !comps) return fail(FTK2H_E_INVALID, "null argument");
+  (void)hip_stream;  // the stream works on its own HIP stream
+  HIP_TRY(hipSetDevice(S->ctx->device));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, S->st));
+  const uint64_t half = S->t % 2;
+  // copy the timestep into its ping-pong half (x fastest, contiguous)
+  for (int c = 0; c < S->ncomp; ++c) {
+    char* dst = static_cast<char*>(S->fields[c]) + half * S->nsp * S->esz;
+    const ftk2h_field_view& v = comps[c];
+    int64_t want = 1;
+    bool contiguous = true;
+    for (int j = 0; j < S->sd; ++j) {
+      contiguous = contiguous && (S->spatial[j] == 1 || v.strides[j] == want);
+      want *= (int64_t)S->spatial[j];
+    }
+    if (contiguous) {
+      HIP_TRY(hipMemcpyAsync(dst, v.ptr, S->nsp * S->esz, v.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             S->st));
+    } else {
+      if (v.on_device) return fail(FTK2H_E_UNSUPPORTED, "stream: device views must be contiguous");
+      std::vector<char> pack(S->nsp * S->esz);
+      int64_t lc[3] = {0, 0, 0};
+      for (uint64_t i = 0; i < S->nsp; ++i) {
+        int64_t idx = 0;
+        for (int j = 0; j < S->sd; ++j) idx += lc[j] * v.strides[j];
+        std::memcpy(&pack[i * S->esz], static_cast<const char*>(v.ptr) + idx * (int64_t)S->esz, S->esz);
+        for (int j = 0; j < S->sd; ++j) {
+          if (++lc[j] < (int64_t)S->spatial[j]) break;
+          lc[j] = 0;
+        }
+      }
+      HIP_TRY(hipMemcpyAsync(dst, pack.data(), pack.size(), hipMemcpyHostToDevice, S->st));
+      HIP_TRY(hipStreamSynchronize(S->st));
+    }
+  }
+  S->t++;
+  if (S->t >= 2) {
+    // slab [t-2, t-1]: views over both halves (time stride = distance between halves)
+    const uint64_t slab_t = S->t - 2;
+    const uint64_t hp = slab_t % 2;
+    ftk2h_mesh_desc md{};
+    md.ndims = S->sd + 1;
+    for (int j = 0; j < 4; ++j) {
+      md.local_dims[j] = j < S->sd ? S->spatial[j] : 1;
+      md.offset[j] = 0;
+      md.global_dims[j] = j < S->sd ? S->spatial[j] : 1;
+    }
+    md.local_dims[S->sd] = 2;
+    md.offset[S->sd] = slab_t;
+    md.global_dims[S->sd] = slab_t + 2;
+    ftk2h_field_view views[3];
+    for (int c = 0; c < S->ncomp; ++c) {
+      views[c] = ftk2h_field_view{};
+      views[c].on_device = 1;
+      views[c].ptr = static_cast<char*>(S->fields[c]) + hp * S->nsp * S->esz;
+      int64_t s = 1;
+      for (int j = 0; j < S->sd; ++j) {
+        views[c].strides[j] = s;
+        s *= (int64_t)S->spatial[j];
+      }
+      views[c].strides[S->sd] = hp == 0 ? (int64_t)S->nsp : -(int64_t)S->nsp;
+    }
+    HIP_TRY(hipStreamSynchronize(S->st));
+    ftk2h_result* R = nullptr;
+    int rc = ftk2h_extract_regular(S->ctx, &md, &S->pred, views, S->ncomp, nullptr, nullptr, &R, S->st);
+    if (rc) return rc;
+    uint32_t nlow = 0;
+    if ((rc = slab_lower_count(S, R, slab_t, &nlow))) {
+      ftk2h_result_free(R);
+      return rc;
+    }
+    if (S->open) {
+      rc = commit_open(S, R, nlow);
+      ftk2h_result_free(S->open);
+      S->open = nullptr;
+      if (rc) {
+        ftk2h_result_free(R);
+        return rc;
+      }
+    }
+    S->open = R;
+    S->open_nlow = nlow;
+  }
+  HIP_TRY(hipEventRecord(e1, S->st));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  S->ms_total += ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return FTK2H_OK;
+}
+
+int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
+  if (!S || !out) return fail(FTK2H_E_INVALID, "null argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(S->ctx->device));
+  int rc;
+  if (S->open) {
+    rc = commit_open(S, nullptr, 0);
+    ftk2h_result_free(S->open);
+    S->open = nullptr;
+    if (rc) return rc;
+  }
+  auto res = std::make_unique<ftk2h_result>();
+  res->device = S->ctx->device;
+  res->kind = S->pred.kind;
+  res->d = S->sd + 1;
+  res->m = S->pred.codimension;
+  res->dtype = S->pred.dtype;
+  for (int j = 0; j < S->sd; ++j) res->gdims[j] = S->spatial[j];
+  res->gdims[S->sd] = S->t;
+  res->ntypes = S->ntypes;
+  const uint64_t n = S->committed;
+  res->n_nodes = n;
+  if (n) {
+    const size_t lsz = (size_t)(res->m + 1) * S->esz;
+    HIP_TRY(hipMalloc(&res->keys, n * 8));
+    HIP_TRY(hipMalloc(&res->lambda, n * lsz));
+    HIP_TRY(hipMalloc(&res->track, n * 4));
+    HIP_TRY(hipMemcpyAsync(res->keys, S->keys.p, n * 8, hipMemcpyDeviceToDevice, S->st));
+    HIP_TRY(hipMemcpyAsync(res->lambda, S->lambda.p, n * lsz, hipMemcpyDeviceToDevice, S->st));
+    k_compress<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(static_cast<uint32_t*>(S->parent.p),
+                                                                         res->track, n);
+    HIP_TRY(hipGetLastError());
+  }
+  if (S->nv && S->committed_cells) {
+    res->n_conn = 1;
+    res->conn_dim[0] = S->nv - 1;
+    res->conn_cells[0] = S->committed_cells;
+    HIP_TRY(hipMalloc(&res->conn[0], S->committed_cells * S->nv * 4));
+    HIP_TRY(hipMemcpyAsync(res->conn[0], S->cells.p, S->committed_cells * S->nv * 4, hipMemcpyDeviceToDevice, S->st));
+    if (S->nv == 3 && S->pred.kind != FTK2H_PRED_CONTOUR) {
+      // star-fan triangles: slab order is not the global order
+      if ((rc = sort_unique_cells<3>(res.get(), n, S->committed_cells, S->st))) return rc;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(S->st));
+  res->phase_ms[0] = S->ms_total;
+  *out = res.release();
+  return FTK2H_OK;
+}
+
+void ftk2h_stream_close(ftk2h_stream* S) {
+  if (!S) return;
+  (void)hipSetDevice(S->ctx->device);
+  if (S->st) (void)hipStreamSynchronize(S->st);
+  if (S->open) ftk2h_result_free(S->open);
+  for (auto* f : S->fields)
+    if (f) (void)hipFree(f);
+  if (S->scratch) (void)hipFree(S->scratch);
+  S->keys.release();
+  S->lambda.release();
+  S->parent.release();
+  S->cells.release();
+  if (S->st) (void)hipStreamDestroy(S->st);
+  delete S;
+}
+
+}  // extern "C"

@@ -280,3 +280,45 @@ def _shifted(a, shift):
         return a
     sl = tuple(slice(int(o), None) for o in shift)
     return a[sl]
+
+
+class Stream:
+    """Timestep-by-timestep extraction (ftk2h_stream_*): execute_stream semantics with
+    the last timestep and the open track frontier resident on the GPU."""
+
+    def __init__(self, spatial_dims, kind: int, m: int, dtype: int, ncomp: int, thresholds=(0.0, 0.0),
+                 sos_q: float = 1e6, device: int = 0):
+        self.spatial = [int(x) for x in spatial_dims]
+        pd = _lib.PredicateDesc()
+        pd.kind = kind
+        pd.codimension = m
+        pd.dtype = dtype
+        pd.sos_q = sos_q
+        pd.thresholds[0] = float(thresholds[0])
+        pd.thresholds[1] = float(thresholds[1]) if len(thresholds) > 1 else 0.0
+        sp = (C.c_uint64 * 3)(*(self.spatial + [1] * (3 - len(self.spatial))))
+        h = C.c_void_p()
+        _lib.check(_lib.lib().ftk2h_stream_open(context(device), len(self.spatial), sp, C.byref(pd), ncomp,
+                                                C.byref(h)))
+        self.h = h.value
+        self.ncomp = ncomp
+
+    def append(self, comps):
+        comps = list(comps)
+        if len(comps) != self.ncomp:
+            raise ValueError("wrong number of components")
+        views = (_lib.FieldView * len(comps))(*[field_view(c) for c in comps])
+        _lib.check(_lib.lib().ftk2h_stream_append(self.h, views, None))
+
+    def finish(self) -> Result:
+        out = C.c_void_p()
+        _lib.check(_lib.lib().ftk2h_stream_finish(self.h, C.byref(out)))
+        return Result(out.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().ftk2h_stream_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,236 @@
+// SimplicialEngine<T, Predicate> of FTK2 (hguo/ftk2 include/ftk2/core/engine.hpp:79-99,
+// :822-996) with its execute() / get_complex() contract, running on an MI355X
+// through the C ABI of include/ftk2h.h (libftk2h.so).  Regular meshes with the
+// critical-point, contour and fiber predicates are supported; anything else throws
+// (there is no CPU implementation behind this header).
+#pragma once
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include <ndarray/ndarray.hh>
+
+#include "../../ftk2h.h"
+#include "complex.hpp"
+#include "feature.hpp"
+#include "mesh.hpp"
+#include "predicate.hpp"
+
+namespace ftk2 {
+
+namespace detail {
+template <class P>
+struct pred_traits {
+  static constexpr bool supported = false;
+};
+template <int M, class T>
+struct pred_traits<CriticalPointPredicate<M, T>> {
+  static constexpr bool supported = true;
+  static constexpr int kind = FTK2H_PRED_CRITICAL_POINT;
+};
+template <class T>
+struct pred_traits<ContourPredicate<T>> {
+  static constexpr bool supported = true;
+  static constexpr int kind = FTK2H_PRED_CONTOUR;
+};
+template <class T>
+struct pred_traits<FiberPredicate<T>> {
+  static constexpr bool supported = true;
+  static constexpr int kind = FTK2H_PRED_FIBER;
+};
+
+inline void check(int rc) {
+  if (rc != FTK2H_OK) throw std::runtime_error(std::string("ftk2h: ") + ftk2h_last_error());
+}
+
+// Process-wide context per device.
+inline ftk2h_ctx* context(int device) {
+  static std::map<int, ftk2h_ctx*> ctxs;
+  auto it = ctxs.find(device);
+  if (it != ctxs.end()) return it->second;
+  ftk2h_ctx* c = nullptr;
+  check(ftk2h_ctx_create(device, &c));
+  ctxs[device] = c;
+  return c;
+}
+}  // namespace detail
+
+template <typename T, typename PredicateType, typename IDType = uint64_t>
+class SimplicialEngine {
+  static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>, "T must be float or double");
+
+ public:
+  static constexpr int m = PredicateType::codimension;
+
+  SimplicialEngine(std::shared_ptr<Mesh> mesh, PredicateType pred = PredicateType(), int device = 0)
+      : mesh_(std::move(mesh)), predicate_(pred), device_(device) {}
+  ~SimplicialEngine() { clear_results(); }
+  SimplicialEngine(const SimplicialEngine&) = delete;
+  SimplicialEngine& operator=(const SimplicialEngine&) = delete;
+
+  void clear_results() {
+    if (result_) ftk2h_result_free(result_);
+    result_ = nullptr;
+  }
+
+  // engine.hpp:96 — extraction over the whole mesh (the C ABI does feed() and the
+  // linking of get_complex() in one pass on the device).
+  void execute(const std::map<std::string, ftk::ndarray<T>>& data, const std::vector<std::string>& var_names = {}) {
+    (void)var_names;  // the predicate's own variable names select the arrays
+    clear_results();
+    auto reg = std::dynamic_pointer_cast<RegularSimplicialMesh>(mesh_);
+    if (!reg) throw std::runtime_error("SimplicialEngine: only RegularSimplicialMesh is supported on the GPU");
+    if constexpr (!detail::pred_traits<PredicateType>::supported) {
+      throw std::runtime_error("SimplicialEngine: predicate not supported on the GPU");
+    } else {
+      if (!predicate_.attributes.empty())
+        throw std::runtime_error("SimplicialEngine: attribute recording is not supported on the GPU path");
+      const auto& ld = reg->get_local_dims();
+      const auto& off = reg->get_offset();
+      const auto& gd = reg->get_global_dims();
+      const int d = (int)ld.size();
+      if (d < 2 || d > 4) throw std::runtime_error("SimplicialEngine: mesh dimension must be 2..4");
+      ftk2h_mesh_desc md{};
+      md.ndims = d;
+      for (int j = 0; j < 4; ++j) {
+        md.local_dims[j] = j < d ? ld[j] : 1;
+        md.offset[j] = j < d ? off[j] : 0;
+        md.global_dims[j] = j < d ? gd[j] : 1;
+      }
+      ftk2h_predicate_desc pd{};
+      pd.kind = detail::pred_traits<PredicateType>::kind;
+      pd.codimension = m;
+      pd.dtype = std::is_same_v<T, double> ? FTK2H_F64 : FTK2H_F32;
+      pd.sos_q = predicate_.sos_q;
+      std::vector<ftk2h_field_view> views;
+      ftk2h_field_view scalar{};
+      bool has_scalar = false;
+      if constexpr (detail::pred_traits<PredicateType>::kind == FTK2H_PRED_CRITICAL_POINT) {
+        if (predicate_.use_multicomponent && !predicate_.vector_var_name.empty()) {
+          // [M, x, y, (z), t] read at GLOBAL coordinates (engine.hpp:1101-1124)
+          const auto& vec = data.at(predicate_.vector_var_name);
+          for (int j = 0; j < m; ++j) views.push_back(component_view(vec, j, gd, off, d));
+        } else {
+          for (int j = 0; j < m; ++j) views.push_back(legacy_view(data.at(predicate_.var_names[j]), reg.get()));
+          // scalar interpolation reads global coordinates (predicate.hpp:81-92)
+          if (!predicate_.scalar_var_name.empty()) {
+            scalar = global_view(data.at(predicate_.scalar_var_name), off, d);
+            has_scalar = true;
+          }
+        }
+      } else if constexpr (detail::pred_traits<PredicateType>::kind == FTK2H_PRED_CONTOUR) {
+        views.push_back(legacy_view(data.at(predicate_.var_name), reg.get()));
+        pd.thresholds[0] = (double)predicate_.threshold;
+      } else {
+        for (int j = 0; j < 2; ++j) {
+          views.push_back(legacy_view(data.at(predicate_.var_names[j]), reg.get()));
+          pd.thresholds[j] = (double)predicate_.thresholds[j];
+        }
+      }
+      detail::check(ftk2h_extract_regular(detail::context(device_), &md, &pd, views.data(), (int)views.size(),
+                                          has_scalar ? &scalar : nullptr, nullptr, &result_, nullptr));
+    }
+  }
+
+  // engine.hpp:822 — vertices sorted by simplex, track_id = minimum sorted index of
+  // the component, connectivity sorted and unique.
+  FeatureComplex get_complex() {
+    FeatureComplex cx;
+    if (!result_) return cx;
+    uint64_t n = 0;
+    int dim = 0, nconn = 0;
+    detail::check(ftk2h_result_sizes(result_, &n, &dim, &nconn));
+    const int nv = dim + 1;
+    std::vector<uint64_t> verts(n * nv), track(n);
+    std::vector<float> bary(n * nv), scal(n);
+    if (n) detail::check(ftk2h_result_copy_nodes(result_, verts.data(), nullptr, bary.data(), scal.data(), track.data()));
+    cx.vertices.resize(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      FeatureElement& el = cx.vertices[i];
+      el.simplex.dimension = dim;
+      for (int k = 0; k < nv; ++k) {
+        el.simplex.vertices[k] = verts[i * nv + k];
+        el.barycentric_coords[0][k] = bary[i * nv + k];
+      }
+      el.geometry_type = FeatureGeometryType::Point;
+      el.track_id = track[i];
+      el.type = 0;
+      el.scalar = scal[i];
+    }
+    for (int c = 0; c < nconn; ++c) {
+      int cd = 0;
+      uint64_t cells = 0;
+      detail::check(ftk2h_result_conn_info(result_, c, &cd, &cells));
+      FeatureComplex::SimplexIndices si;
+      si.dimension = cd;
+      si.indices.resize(cells * (cd + 1));
+      if (cells) detail::check(ftk2h_result_copy_conn(result_, c, si.indices.data()));
+      cx.connectivity.push_back(std::move(si));
+    }
+    return cx;
+  }
+
+  // Device-side result of the last execute() (for callers that keep working on the GPU).
+  const ftk2h_result* result() const { return result_; }
+
+ private:
+  static ftk2h_field_view legacy_view(const ftk::ndarray<T>& arr, const RegularSimplicialMesh* reg) {
+    // get_value (engine.hpp:999-1009): 1-D arrays by vertex id, else f(local coords)
+    ftk2h_field_view v{};
+    v.on_device = 0;
+    const int d = (int)reg->get_local_dims().size();
+    if (arr.nd() == 1) {
+      const auto& gd = reg->get_global_dims();
+      const auto& off = reg->get_offset();
+      int64_t s = 1, base = 0;
+      for (int j = 0; j < d; ++j) {
+        v.strides[j] = s;
+        base += (int64_t)off[j] * s;
+        s *= (int64_t)gd[j];
+      }
+      v.ptr = arr.data() + base;
+    } else {
+      int64_t s = 1;
+      for (int j = 0; j < d; ++j) {
+        v.strides[j] = s;
+        s *= (int64_t)arr.dimf(j);
+      }
+      v.ptr = arr.data();
+    }
+    return v;
+  }
+  static ftk2h_field_view global_view(const ftk::ndarray<T>& arr, const std::vector<uint64_t>& off, int d) {
+    ftk2h_field_view v{};
+    int64_t s = 1, base = 0;
+    for (int j = 0; j < d; ++j) {
+      v.strides[j] = s;
+      base += (int64_t)off[j] * s;
+      s *= (int64_t)arr.dimf(j);
+    }
+    v.ptr = arr.data() + base;
+    return v;
+  }
+  static ftk2h_field_view component_view(const ftk::ndarray<T>& vec, int comp, const std::vector<uint64_t>& gd,
+                                         const std::vector<uint64_t>& off, int d) {
+    (void)gd;
+    ftk2h_field_view v{};
+    int64_t s = (int64_t)vec.dimf(0), base = comp;
+    for (int j = 0; j < d; ++j) {
+      v.strides[j] = s;
+      base += (int64_t)off[j] * s;
+      s *= (int64_t)vec.dimf(j + 1);
+    }
+    v.ptr = vec.data() + base;
+    return v;
+  }
+
+  std::shared_ptr<Mesh> mesh_;
+  PredicateType predicate_;
+  int device_ = 0;
+  ftk2h_result* result_ = nullptr;
+};
+
+}  // namespace ftk2

@@ -130,6 +130,19 @@ int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out);
 int ftk2h_result_phase_ms(const ftk2h_result* r, float* ms, int n);
 void ftk2h_result_free(ftk2h_result* r);
 
+/* ---- Streaming (SimplicialEngine::execute_stream, engine.hpp:214-270) ----------
+ * Timesteps of a spatial grid arrive one at a time (ftk2h_stream_append, views
+ * indexed by spatial coordinates).  Each append of timestep t >= 1 extracts the
+ * slab [t-1, t] on the device; the last timestep and the open union-find frontier
+ * stay resident.  ftk2h_stream_finish returns the complex over all timesteps, the
+ * same as one ftk2h_extract_regular over the whole spacetime grid. */
+typedef struct ftk2h_stream ftk2h_stream;
+int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial_dims,
+                      const ftk2h_predicate_desc* pred, int ncomp, ftk2h_stream** out);
+int ftk2h_stream_append(ftk2h_stream* s, const ftk2h_field_view* comps, void* hip_stream);
+int ftk2h_stream_finish(ftk2h_stream* s, ftk2h_result** out);
+void ftk2h_stream_close(ftk2h_stream* s);
+
 const char* ftk2h_last_error(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,122 @@
+// End-to-end cases of the reference's tests/test_examples.cpp, compiled against
+// the drop-in headers (include/ftk2/core/*.hpp) and run on the GPU through
+// libftk2h.so.  Exit code = number of failed checks.
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <vector>
+
+#include <ftk2/core/engine.hpp>
+
+using namespace ftk2;
+
+static int failures = 0;
+#define CHECK_EQ(a, b)                                                                      \
+  do {                                                                                      \
+    auto va = (a);                                                                          \
+    auto vb = (b);                                                                          \
+    if (!(va == vb)) {                                                                      \
+      std::printf("FAILED %s == %s (%lld vs %lld) at line %d\n", #a, #b, (long long)va,   \
+                  (long long)vb, __LINE__);                                                 \
+      failures++;                                                                           \
+    }                                                                                       \
+  } while (0)
+
+// vertices sorted by simplex, cells sorted/unique, track ids = component minima
+static void check_complex(const FeatureComplex& cx) {
+  for (size_t i = 1; i < cx.vertices.size(); ++i) CHECK_EQ(cx.vertices[i - 1].simplex < cx.vertices[i].simplex, true);
+  for (size_t i = 0; i < cx.vertices.size(); ++i) CHECK_EQ(cx.vertices[i].track_id <= i, true);
+  for (const auto& c : cx.connectivity) {
+    const int n = c.dimension + 1;
+    for (size_t i = 0; i + n <= c.indices.size(); i += n) {
+      for (int q = 1; q < n; ++q) {
+        CHECK_EQ(c.indices[i + q - 1] < c.indices[i + q], true);
+        CHECK_EQ(cx.vertices[c.indices[i]].track_id, cx.vertices[c.indices[i + q]].track_id);
+      }
+    }
+  }
+}
+
+static void levelset_3d() {  // test_examples.cpp:229-257
+  const int W = 32, H = 32, Dd = 32, Tn = 10;
+  ftk::ndarray<float> s({(size_t)W, (size_t)H, (size_t)Dd, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t) {
+    const float cx = 16.0f + t * 0.2f;
+    for (int z = 0; z < Dd; ++z)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+          const float d = std::sqrt(std::pow(x - cx, 2) + std::pow(y - 16.0f, 2) + std::pow(z - 16.0f, 2));
+          s.f(x, y, z, t) = d - 8.0f;
+        }
+  }
+  std::map<std::string, ftk::ndarray<float>> data = {{"S", s}};
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{32, 32, 32, 10});
+  ContourPredicate<float> pred;
+  pred.var_name = "S";
+  SimplicialEngine<float, ContourPredicate<float>> engine(mesh, pred);
+  engine.execute(data, {"S"});
+  auto cx = engine.get_complex();
+  CHECK_EQ(cx.vertices.size(), (size_t)66418);
+  check_complex(cx);
+  std::printf("levelset_3d: %zu vertices\n", cx.vertices.size());
+}
+
+static void fiber_3d() {  // test_examples.cpp:188-227
+  const int W = 32, H = 32, Dd = 32, Tn = 10;
+  ftk::ndarray<double> s1({(size_t)W, (size_t)H, (size_t)Dd, (size_t)Tn}), s2({(size_t)W, (size_t)H, (size_t)Dd, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t)
+    for (int z = 0; z < Dd; ++z)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+          const double a = std::sqrt(std::pow(x - 16.0, 2) + std::pow(y - 16.0, 2) + std::pow(z - 16.0, 2));
+          const double b = std::sqrt(std::pow(x - 22.0, 2) + std::pow(y - (16.0 + t * 0.5), 2) + std::pow(z - 16.0, 2));
+          s1.f(x, y, z, t) = a - 10.0;
+          s2.f(x, y, z, t) = b - 8.0;
+        }
+  std::map<std::string, ftk::ndarray<double>> data = {{"S1", s1}, {"S2", s2}};
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{32, 32, 32, 10});
+  FiberPredicate<double> pred;
+  pred.var_names[0] = "S1";
+  pred.var_names[1] = "S2";
+  SimplicialEngine<double, FiberPredicate<double>> engine(mesh, pred);
+  engine.execute(data, {"S1", "S2"});
+  auto cx = engine.get_complex();
+  CHECK_EQ(cx.vertices.size(), (size_t)6078);
+  check_complex(cx);
+  std::printf("fiber_3d: %zu vertices\n", cx.vertices.size());
+}
+
+static void critical_point_3d() {  // test_examples.cpp:153-186 (see tests/test_oracle_engine.py)
+  const int W = 24, H = 24, Dd = 24, Tn = 8;
+  ftk::ndarray<float> u({(size_t)W, (size_t)H, (size_t)Dd, (size_t)Tn}), v(u), w(u);
+  for (int t = 0; t < Tn; ++t) {
+    const float c = 16.0f + t * 0.1f;
+    for (int z = 0; z < Dd; ++z)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+          u.f(x, y, z, t) = (float)x - c;
+          v.f(x, y, z, t) = (float)y - 16.0f;
+          w.f(x, y, z, t) = (float)z - 16.0f;
+        }
+  }
+  std::map<std::string, ftk::ndarray<float>> data = {{"U", u}, {"V", v}, {"W", w}};
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{24, 24, 24, 8});
+  CriticalPointPredicate<3, float> pred;
+  pred.var_names[0] = "U";
+  pred.var_names[1] = "V";
+  pred.var_names[2] = "W";
+  SimplicialEngine<float, CriticalPointPredicate<3, float>> engine(mesh, pred);
+  engine.execute(data, {"U", "V", "W"});
+  auto cx = engine.get_complex();
+  CHECK_EQ(cx.vertices.size(), (size_t)29);
+  check_complex(cx);
+  std::printf("critical_point_3d: %zu vertices\n", cx.vertices.size());
+}
+
+int main() {
+  levelset_3d();
+  fiber_3d();
+  critical_point_3d();
+  std::printf("%d failures\n", failures);
+  return failures;
+}

@@ -1,0 +1,71 @@
+"""Streaming extraction on the GPU: appending timesteps one at a time gives the same
+complex as the one-shot extraction (test_examples.cpp:259-297 contract) and as the
+oracle's execute_stream restatement."""
+import numpy as np
+import pytest
+
+import oracle
+from ftk2_amd import _lib, engine, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def run_stream(comps, kind, m, thr=(0.0, 0.0)):
+    nt = comps[0].shape[-1]
+    dt = _lib.F64 if comps[0].dtype == np.float64 else _lib.F32
+    s = engine.Stream(comps[0].shape[:-1], kind, m, dt, len(comps), thr)
+    for t in range(nt):
+        s.append([np.asfortranarray(c[..., t]) for c in comps])
+    r = s.finish().to_complex()
+    s.close()
+    return r
+
+
+def same(a, b):
+    assert np.array_equal(a.vertices, b.vertices)
+    assert np.array_equal(a.lam.view(np.uint64), b.lam.view(np.uint64))
+    assert np.array_equal(a.track_id, b.track_id)
+    assert sorted(a.connectivity) == sorted(b.connectivity)
+    for d in a.connectivity:
+        assert np.array_equal(a.connectivity[d], b.connectivity[d])
+
+
+@pytest.mark.parametrize("case", ["cp3d", "cp2d", "contour", "fiber", "cp3d_float"])
+def test_stream_equals_oneshot(case):
+    print(f"[stream] {case}", flush=True)
+    if case == "cp3d":
+        comps, kind, m, thr = synth.cp3d_config(n=18, nt=6, seed=3), 0, 3, (0, 0)
+    elif case == "cp3d_float":
+        comps = synth.gaussians((16, 14, 12), 5, 6, sigma=(8, 24), seed=4, noise=1e-3, zero_frac=0.02,
+                                dup_frac=0.02, dtype=np.float32)
+        kind, m, thr = 0, 3, (0, 0)
+    elif case == "cp2d":
+        comps, kind, m, thr = list(synth.cp2d_config(nx=40, ny=36, nt=7, seed=2)), 0, 2, (0, 0)
+    elif case == "contour":
+        comps = [synth.scalar_config(n=16, nt=6, ngauss=8, seed=1, dtype=np.float64)]
+        kind, m, thr = 1, 1, (0.0, 0)
+    else:
+        comps = [synth.scalar_config(n=14, nt=5, ngauss=8, seed=s, dtype=np.float64) for s in (7, 8)]
+        kind, m, thr = 2, 2, (0.0, 0.0)
+    mesh = engine.RegularSimplicialMesh(comps[0].shape)
+    one = engine.extract(mesh, kind, m, comps, thr).to_complex()
+    st = run_stream(comps, kind, m, thr)
+    same(st, one)
+    # and the oracle's execute_stream
+    steps = [[np.asfortranarray(c[..., t]) for c in comps] for t in range(comps[0].shape[-1])]
+    ref = oracle.run_stream(steps, kind, m, thr=thr)
+    assert np.array_equal(st.vertices, ref["verts"])
+    assert np.array_equal(st.track_id, ref["track"])
+    for d, cells in ref["conn"].items():
+        assert np.array_equal(st.connectivity[d], cells)
+
+
+def test_stream_two_timesteps_and_empty():
+    comps = synth.cp3d_config(n=10, nt=2, seed=5)
+    mesh = engine.RegularSimplicialMesh(comps[0].shape)
+    same(run_stream(comps, 0, 3), engine.extract(mesh, 0, 3, comps).to_complex())
+    s = engine.Stream([10, 10, 10], 0, 3, _lib.F64, 3)
+    s.append([np.asfortranarray(c[..., 0]) for c in comps])
+    r = s.finish()
+    assert r.sizes()[0] == 0
+    s.close()
