@@ -22,6 +22,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 N_TYPES = 60  # tetrahedra per 4-D hypercube
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "cp3d_256x4_fp64_hbm_traffic.md")
+
+
+def classify_traffic(n, nt):
+    """HBM bytes per k_classify launch measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes recorded in the committed summary (256^3 x 4 only; None otherwise)."""
+    if (n, nt) != (256, 4) or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    import re
+    m = re.search(r"^classify_traffic_bytes:\s*(\d+)", open(TRAFFIC_SUMMARY).read(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def log(*a):
@@ -56,7 +67,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nt", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (PMC calibration)")
@@ -143,7 +154,9 @@ def main():
                    "mesh": [n, n, n, nt], "gaussians": 32, "noise": 1e-3, "zero_frac": 0.01,
                    "dup_frac": 0.01, "seed": args.seed, "nodes": int(n_nodes), "edges": int(n_edges)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": classify_traffic(n, nt) if dominant == "classify" else None,
+                     "algorithmic_bytes": int(dom_bytes)},
         "pipeline": {"kernel_ms": kernel_ms, "phase_ms": [float(x) for x in ph],
                      "phases": ["classify", "scan_nodes", "offsets+cell_count", "scan_cells+sync",
                                 "emit", "link", "compress"],
