@@ -48,6 +48,14 @@ class DeviceView(C.Structure):
                 ("conn", C.c_void_p * 3)]
 
 
+class Job(C.Structure):
+    _fields_ = [("mesh", MeshDesc), ("comps", C.POINTER(FieldView)), ("scalar", C.POINTER(FieldView))]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("grid_per_cu", C.c_int), ("chunk", C.c_int), ("use_graph", C.c_int)]
+
+
 # Every entry point of include/ftk2h.h: name -> (restype, argtypes)
 _P = C.c_void_p
 SIGNATURES = {
@@ -70,6 +78,14 @@ SIGNATURES = {
     "ftk2h_stream_append": (C.c_int, [_P, C.POINTER(FieldView), _P]),
     "ftk2h_stream_finish": (C.c_int, [_P, C.POINTER(_P)]),
     "ftk2h_stream_close": (None, [_P]),
+    "ftk2h_batch_create": (C.c_int, [_P, C.POINTER(PredicateDesc), C.c_int, C.POINTER(_P)]),
+    "ftk2h_batch_set_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
+    "ftk2h_batch_get_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
+    "ftk2h_batch_run": (C.c_int, [_P, C.c_int, C.POINTER(Job), C.POINTER(_P), _P]),
+    "ftk2h_batch_autotune": (C.c_int, [_P, C.c_int, C.POINTER(Job), C.c_int, C.POINTER(BatchConfig),
+                                       C.POINTER(C.c_float)]),
+    "ftk2h_batch_last_ms": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
+    "ftk2h_batch_destroy": (None, [_P]),
 }
 
 _lib = None

@@ -133,6 +133,7 @@ void ftk2h_ctx_destroy(ftk2h_ctx* ctx) {
   ctx->ebsum.release();
   ctx->xedges.release();
   ctx->xcount.release();
+  ctx->scan_sums.release();
   for (auto& f : ctx->fields) f.release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -148,35 +149,17 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   const int d = mesh->ndims;
   if (d < 2 || d > 4) return fail(FTK2H_E_UNSUPPORTED, "ndims must be 2..4");
   const int m = pred->codimension;
-  int need = 0;
-  switch (pred->kind) {
-    case FTK2H_PRED_CRITICAL_POINT:
-      need = m;
-      break;
-    case FTK2H_PRED_CONTOUR:
-      need = 1;
-      if (m != 1) return fail(FTK2H_E_INVALID, "contour codimension must be 1");
-      break;
-    case FTK2H_PRED_FIBER:
-      need = 2;
-      if (m != 2) return fail(FTK2H_E_INVALID, "fiber codimension must be 2");
-      break;
-    default:
-      return fail(FTK2H_E_INVALID, "unknown predicate kind");
-  }
+  const int need = components_needed(pred);
+  if (need < 0) return FTK2H_E_INVALID;
   if (pred->dtype != FTK2H_F32 && pred->dtype != FTK2H_F64) return fail(FTK2H_E_INVALID, "bad dtype");
   if (ncomp != need || !comps) return fail(FTK2H_E_INVALID, "wrong number of field components");
   RunFn fn = find_entry(pred->kind, d, m, pred->dtype);
   if (!fn)
     return fail(FTK2H_E_UNSUPPORTED, "predicate kind " + std::to_string(pred->kind) + " with d=" +
                                          std::to_string(d) + " m=" + std::to_string(m) + " not built");
-  uint64_t nvert = 1;
-  for (int j = 0; j < d; ++j) {
-    if (mesh->local_dims[j] == 0) return fail(FTK2H_E_INVALID, "zero extent");
-    if (mesh->global_dims[j] < mesh->local_dims[j] + mesh->offset[j])
-      return fail(FTK2H_E_INVALID, "local dims + offset exceed global dims");
-    nvert *= mesh->local_dims[j];
-  }
+  uint64_t nvert = 0;
+  bool has_cells = false;
+  if (int rc = check_mesh(mesh, d, &nvert, &has_cells)) return rc;
   for (int c = 0; c < ncomp; ++c)
     if (!comps[c].ptr) return fail(FTK2H_E_INVALID, "null field pointer");
 
@@ -191,32 +174,14 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   res->m = m;
   res->dtype = pred->dtype;
   for (int j = 0; j < d; ++j) res->gdims[j] = mesh->global_dims[j];
-
-  bool has_cells = true;
-  for (int j = 0; j < d; ++j) has_cells = has_cells && mesh->local_dims[j] >= 2;
   if (!has_cells) {  // no hypercube: the reference visits nothing
     *out = res.release();
     return FTK2H_OK;
   }
 
   KParams P;
-  std::memset(&P, 0, sizeof(P));
-  for (int j = 0; j < 4; ++j) {
-    P.dims[j] = j < d ? (int64_t)mesh->local_dims[j] : 1;
-    P.off[j] = j < d ? (int64_t)mesh->offset[j] : 0;
-  }
-  int64_t gs = 1;
-  for (int j = 0; j < d; ++j) {
-    P.gstride[j] = gs;
-    gs *= (int64_t)mesh->global_dims[j];
-  }
-  P.nvert = (int64_t)nvert;
-  P.small = nvert < (1ull << 31);
-  P.Q = pred->sos_q;
-  P.thr[0] = pred->thresholds[0];
-  P.thr[1] = pred->thresholds[1];
+  init_params(P, mesh, pred, nvert);
   const size_t esz = pred->dtype == FTK2H_F64 ? 8 : 4;
-
   // Field views: device views are read in place; host views are packed into a
   // contiguous (x fastest) device copy.
   auto bind = [&](const ftk2h_field_view& fv, int slot, const void** ptr, int64_t* strides) -> int {
@@ -226,27 +191,13 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
       return FTK2H_OK;
     }
     std::vector<char> pack(nvert * esz);
-    const char* src = static_cast<const char*>(fv.ptr);
-    int64_t lc[4] = {0, 0, 0, 0};
-    for (uint64_t v = 0; v < nvert; ++v) {
-      int64_t idx = 0;
-      for (int j = 0; j < d; ++j) idx += lc[j] * fv.strides[j];
-      std::memcpy(&pack[v * esz], src + idx * (int64_t)esz, esz);
-      for (int j = 0; j < d; ++j) {
-        if (++lc[j] < (int64_t)mesh->local_dims[j]) break;
-        lc[j] = 0;
-      }
-    }
+    pack_host_view(fv, mesh, esz, nvert, pack.data());
     int rc = ctx->fields[slot].ensure(nvert * esz);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->fields[slot].p, pack.data(), nvert * esz, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     *ptr = ctx->fields[slot].p;
-    int64_t s = 1;
-    for (int j = 0; j < 4; ++j) {
-      strides[j] = j < d ? s : 0;
-      if (j < d) s *= (int64_t)mesh->local_dims[j];
-    }
+    packed_strides(mesh, strides);
     return FTK2H_OK;
   };
   for (int c = 0; c < ncomp; ++c) {
@@ -257,52 +208,14 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
     int rc = bind(*scalar, 3, &P.scal, P.sst);
     if (rc) return rc;
   }
-
-  // per-corner offset tables and the packed-layout flag
-  {
-    int64_t lst[4] = {0, 0, 0, 0}, ls = 1;
-    for (int j = 0; j < d; ++j) {
-      lst[j] = ls;
-      ls *= (int64_t)mesh->local_dims[j];
-    }
-    P.packed = 1;
-    for (int c = 0; c < ncomp; ++c)
-      for (int j = 0; j < d; ++j)
-        if (P.cst[c][j] != lst[j]) P.packed = 0;
-    for (int s = 0; s < 16; ++s) {
-      int64_t lo = 0, so = 0;
-      uint64_t go = 0;
-      int64_t co[3] = {0, 0, 0};
-      for (int j = 0; j < d; ++j) {
-        if (!((s >> j) & 1)) continue;
-        lo += lst[j];
-        go += (uint64_t)P.gstride[j];
-        so += P.sst[j];
-        for (int c = 0; c < 3; ++c) co[c] += P.cst[c][j];
-      }
-      P.loff[s] = lo;
-      P.goff[s] = go;
-      P.soff[s] = so;
-      for (int c = 0; c < 3; ++c) P.coff[c][s] = co[c];
-    }
-  }
-  // Workgroup staging capacities of the emit kernel.  FTK2H_EMIT_CAPS="nq,words"
-  // lowers them (tests use it to drive the overflow paths).
-  P.nq_cap = EMIT_NQ;
-  P.stage_cap = EMIT_STAGE_WORDS;
-  if (const char* caps = std::getenv("FTK2H_EMIT_CAPS")) {
-    unsigned a = 0, b = 0;
-    if (std::sscanf(caps, "%u,%u", &a, &b) == 2) {
-      P.nq_cap = a < (unsigned)EMIT_NQ ? a : EMIT_NQ;
-      P.stage_cap = b < (unsigned)EMIT_STAGE_WORDS ? b : EMIT_STAGE_WORDS;
-    }
-  }
+  finish_params(P, mesh, ncomp);
   const int64_t nb = (int64_t)((nvert + BLOCK - 1) / BLOCK);
   int rc;
   if ((rc = ctx->mask.ensure(nvert * 8))) return rc;
   if ((rc = ctx->node_off.ensure(nvert * 4))) return rc;
   if ((rc = ctx->bsum.ensure((nb + 1) * 4))) return rc;
   if ((rc = ctx->ebsum.ensure((nb + 1) * 4))) return rc;
+  if ((rc = ctx->scan_sums.ensure((nb / SCAN_TILE + 2) * 4))) return rc;
   P.mask = static_cast<uint64_t*>(ctx->mask.p);
   P.node_off = static_cast<uint32_t*>(ctx->node_off.p);
   P.bsum = static_cast<uint32_t*>(ctx->bsum.p);

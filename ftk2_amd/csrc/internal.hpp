@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -16,6 +17,7 @@
 #include "sortcells.hpp"
 
 #include <algorithm>
+#include <functional>
 
 namespace ftk2h_internal {
 using namespace ftk2h;
@@ -68,7 +70,7 @@ constexpr int NPHASE = 7;
 struct ftk2h_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  ftk2h_internal::DevBuf mask, node_off, bsum, ebsum, xedges, xcount, fields[4];
+  ftk2h_internal::DevBuf mask, node_off, bsum, ebsum, xedges, xcount, scan_sums, fields[4];
   hipEvent_t ev[ftk2h_internal::NPHASE + 1] = {};
   std::mutex lock;
 };
@@ -88,7 +90,12 @@ struct ftk2h_result {
   uint64_t conn_cells[3] = {0, 0, 0};
   uint32_t* conn[3] = {nullptr, nullptr, nullptr};
   float phase_ms[ftk2h_internal::NPHASE] = {};
+  bool has_scalar = false;
+  // Set when the arrays are views into memory shared with other results (batched
+  // runs); the memory is released with its last holder.
+  std::shared_ptr<void> arena;
   ~ftk2h_result() {
+    if (arena) return;
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
@@ -115,6 +122,131 @@ struct RunArgs {
 };
 
 using RunFn = int (*)(RunArgs&);
+
+// ---- request checks and kernel parameters shared by every entry point ----------
+
+// Number of field components the predicate reads, or -1 (message set) if the
+// (kind, codimension) pair is invalid.
+inline int components_needed(const ftk2h_predicate_desc* pred) {
+  switch (pred->kind) {
+    case FTK2H_PRED_CRITICAL_POINT:
+      return pred->codimension;
+    case FTK2H_PRED_CONTOUR:
+      if (pred->codimension != 1) return fail(FTK2H_E_INVALID, "contour codimension must be 1"), -1;
+      return 1;
+    case FTK2H_PRED_FIBER:
+      if (pred->codimension != 2) return fail(FTK2H_E_INVALID, "fiber codimension must be 2"), -1;
+      return 2;
+    default:
+      return fail(FTK2H_E_INVALID, "unknown predicate kind"), -1;
+  }
+}
+
+// Validates a mesh descriptor against the predicate's dimension d; returns the
+// vertex count through nvert and whether the mesh has any hypercube.
+inline int check_mesh(const ftk2h_mesh_desc* mesh, int d, uint64_t* nvert, bool* has_cells) {
+  if (mesh->ndims != d) return fail(FTK2H_E_INVALID, "mesh ndims differs from the predicate's");
+  uint64_t n = 1;
+  bool hc = true;
+  for (int j = 0; j < d; ++j) {
+    if (mesh->local_dims[j] == 0) return fail(FTK2H_E_INVALID, "zero extent");
+    if (mesh->global_dims[j] < mesh->local_dims[j] + mesh->offset[j])
+      return fail(FTK2H_E_INVALID, "local dims + offset exceed global dims");
+    n *= mesh->local_dims[j];
+    hc = hc && mesh->local_dims[j] >= 2;
+  }
+  *nvert = n;
+  *has_cells = hc;
+  return FTK2H_OK;
+}
+
+// Mesh and predicate fields of P (field pointers, tables and work buffers excluded).
+inline void init_params(KParams& P, const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred, uint64_t nvert) {
+  std::memset(&P, 0, sizeof(P));
+  const int d = mesh->ndims;
+  for (int j = 0; j < 4; ++j) {
+    P.dims[j] = j < d ? (int64_t)mesh->local_dims[j] : 1;
+    P.off[j] = j < d ? (int64_t)mesh->offset[j] : 0;
+  }
+  int64_t gs = 1;
+  for (int j = 0; j < d; ++j) {
+    P.gstride[j] = gs;
+    gs *= (int64_t)mesh->global_dims[j];
+  }
+  P.nvert = (int64_t)nvert;
+  P.small = nvert < (1ull << 31);
+  P.Q = pred->sos_q;
+  P.thr[0] = pred->thresholds[0];
+  P.thr[1] = pred->thresholds[1];
+}
+
+// Host view -> contiguous x-fastest copy of the local grid.
+inline void pack_host_view(const ftk2h_field_view& fv, const ftk2h_mesh_desc* mesh, size_t esz, uint64_t nvert,
+                           char* dst) {
+  const int d = mesh->ndims;
+  const char* src = static_cast<const char*>(fv.ptr);
+  int64_t lc[4] = {0, 0, 0, 0};
+  for (uint64_t v = 0; v < nvert; ++v) {
+    int64_t idx = 0;
+    for (int j = 0; j < d; ++j) idx += lc[j] * fv.strides[j];
+    std::memcpy(dst + v * esz, src + idx * (int64_t)esz, esz);
+    for (int j = 0; j < d; ++j) {
+      if (++lc[j] < (int64_t)mesh->local_dims[j]) break;
+      lc[j] = 0;
+    }
+  }
+}
+
+// Element strides of a contiguous x-fastest copy of the local grid.
+inline void packed_strides(const ftk2h_mesh_desc* mesh, int64_t* strides) {
+  int64_t s = 1;
+  for (int j = 0; j < 4; ++j) {
+    strides[j] = j < mesh->ndims ? s : 0;
+    if (j < mesh->ndims) s *= (int64_t)mesh->local_dims[j];
+  }
+}
+
+// Per-corner offset tables, the packed-layout flag and the emit capacities, once
+// the component and scalar views of P are bound.
+inline void finish_params(KParams& P, const ftk2h_mesh_desc* mesh, int ncomp) {
+  const int d = mesh->ndims;
+  int64_t lst[4] = {0, 0, 0, 0}, ls = 1;
+  for (int j = 0; j < d; ++j) {
+    lst[j] = ls;
+    ls *= (int64_t)mesh->local_dims[j];
+  }
+  P.packed = 1;
+  for (int c = 0; c < ncomp; ++c)
+    for (int j = 0; j < d; ++j)
+      if (P.cst[c][j] != lst[j]) P.packed = 0;
+  for (int s = 0; s < 16; ++s) {
+    int64_t lo = 0, so = 0;
+    uint64_t go = 0;
+    int64_t co[3] = {0, 0, 0};
+    for (int j = 0; j < d; ++j) {
+      if (!((s >> j) & 1)) continue;
+      lo += lst[j];
+      go += (uint64_t)P.gstride[j];
+      so += P.sst[j];
+      for (int c = 0; c < 3; ++c) co[c] += P.cst[c][j];
+    }
+    P.loff[s] = lo;
+    P.goff[s] = go;
+    P.soff[s] = so;
+    for (int c = 0; c < 3; ++c) P.coff[c][s] = co[c];
+  }
+  // Workgroup staging capacities of the emit kernel.  FTK2H_EMIT_CAPS="nq,words"
+  // lowers them (tests use it to drive the overflow paths).
+  P.nq_cap = EMIT_NQ;
+  P.stage_cap = EMIT_STAGE_WORDS;
+  if (const char* caps = std::getenv("FTK2H_EMIT_CAPS")) {
+    unsigned a = 0, b = 0;
+    if (std::sscanf(caps, "%u,%u", &a, &b) == 2) {
+      P.nq_cap = a < (unsigned)EMIT_NQ ? a : EMIT_NQ;
+      P.stage_cap = b < (unsigned)EMIT_STAGE_WORDS ? b : EMIT_STAGE_WORDS;
+    }
+  }
+}
 
 // Sort + unique the cells of R->conn[0] (bucket by smallest index, sort inside
 // buckets, compact).  Replaces R->conn[0] and R->conn_cells[0].
@@ -180,13 +312,13 @@ int run_pipeline(RunArgs& A) {
   if (nb > 0) k_classify<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(1));
-  k_scan_small<<<1, 1024, 0, st>>>(P.bsum, nb);
+  HIP_TRY(scan_u32(P.bsum, P.bsum, (uint64_t)nb, static_cast<uint32_t*>(ctx->scan_sums.p), st));
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(2));
   if (nb > 0) k_offsets<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(3));
-  k_scan_small<<<1, 1024, 0, st>>>(P.ebsum, nb);
+  HIP_TRY(scan_u32(P.ebsum, P.ebsum, (uint64_t)nb, static_cast<uint32_t*>(ctx->scan_sums.p), st));
   HIP_TRY(hipGetLastError());
   uint32_t totals[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(&totals[0], P.bsum + nb, 4, hipMemcpyDeviceToHost, st));

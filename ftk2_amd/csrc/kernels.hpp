@@ -883,8 +883,11 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
 
 // ---------------------------------------------------------------------------
 // Kernels
+// Tile functions: one workgroup processes the 256 vertices [blk*256, blk*256 + 256)
+// of the mesh described by P.  The single-mesh kernels run one tile per workgroup;
+// the batched kernels (batch.hip) loop over tiles of many meshes.
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK, 3) void k_classify(KParams P) {
+__device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
   constexpr int NTY = Chains<D, M>::N;
   constexpr int QBYTES = BLOCK * NTY * 2;
   constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 0;
@@ -896,7 +899,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_classify(KParams P) {
   __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t qn;
   if (threadIdx.x == 0) qn = 0;
-  const int64_t v0 = (int64_t)blockIdx.x * BLOCK;
+  const int64_t v0 = blk * BLOCK;
   const int64_t v = v0 + threadIdx.x;
   if constexpr (M == 3) stage_quantized3<T, D>(P, v0, tile);
   __syncthreads();
@@ -933,7 +936,12 @@ __global__ __launch_bounds__(BLOCK, 3) void k_classify(KParams P) {
   if (v < P.nvert) P.mask[v] = m;
   uint32_t tot;
   block_excl_scan((uint32_t)__popcll(m), &tot);
-  if (threadIdx.x == 0) P.bsum[blockIdx.x] = tot;
+  if (threadIdx.x == 0) P.bsum[blk] = tot;
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK, 3) void k_classify(KParams P) {
+  classify_tile<KIND, D, M, T>(P, blockIdx.x);
 }
 
 // exclusive scan of n uint32 in place (single block of 1024 threads); total at a[n]
@@ -970,8 +978,8 @@ __device__ __forceinline__ bool is_hc_base(const KParams& P, const int64_t* l) {
 }
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
-  const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void offsets_tile(const KParams& P, int64_t blk) {
+  const int64_t v = blk * BLOCK + threadIdx.x;
   uint64_t m = 0;
   uint32_t nc = 0;
   if (v < P.nvert) {
@@ -983,10 +991,15 @@ __global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
   }
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
-  if (v < P.nvert) P.node_off[v] = P.bsum[blockIdx.x] + ex;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *P.xcount = 0;
+  if (v < P.nvert) P.node_off[v] = P.bsum[blk] + ex;
+  if (blk == 0 && threadIdx.x == 0) *P.xcount = 0;
   block_excl_scan(nc, &tot);
-  if (threadIdx.x == 0) P.ebsum[blockIdx.x] = tot;
+  if (threadIdx.x == 0) P.ebsum[blk] = tot;
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
+  offsets_tile<KIND, D, M, T>(P, blockIdx.x);
 }
 
 // Lexicographic insertion sort of n cells of NV indices (small n).
@@ -1054,13 +1067,13 @@ __device__ __forceinline__ void lds_unite(uint32_t* p, uint32_t a, uint32_t b) {
 constexpr int EMIT_STAGE_WORDS = 4096;  // staged cell indices per workgroup
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
+__device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   constexpr int NTY = Chains<D, M>::N;
   constexpr int NV = CellKind<KIND, D, M>::verts;
   __shared__ uint16_t nq[EMIT_NQ];
   __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t stage[EMIT_STAGE_WORDS];
-  const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t v = blk * BLOCK + threadIdx.x;
   Vtx x;
   uint64_t m = 0;
   bool hc = false;
@@ -1071,8 +1084,8 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
     refs[threadIdx.x] = x.r;
   }
   // ---- nodes
-  const uint32_t nbase = P.bsum[blockIdx.x];
-  const uint32_t ntot = P.bsum[blockIdx.x + 1] - nbase;
+  const uint32_t nbase = P.bsum[blk];
+  const uint32_t ntot = P.bsum[blk + 1] - nbase;
   T* lam_out = static_cast<T*>(P.lambda);
   auto emit_node = [&](uint32_t pos, const VtxRef& r, int t) {
     T lam[M + 1];
@@ -1114,7 +1127,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
     if (hc) nc = cells_of<KIND, D, M>(P, x, 0, nullptr);
     uint32_t ctot;
     const uint32_t ex = block_excl_scan(nc, &ctot);
-    const uint64_t cbase = P.ebsum[blockIdx.x];
+    const uint64_t cbase = P.ebsum[blk];
     if (ctot * NV <= P.stage_cap) {
       if (nc) {
         uint32_t* seg = stage + ex * NV;
@@ -1191,6 +1204,11 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
   }
   if (!linked)
     for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) P.parent[nbase + i] = nbase + i;
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
+  emit_tile<KIND, D, M, T>(P, blockIdx.x);
 }
 
 // union-find hooking (larger root under smaller) over every cell

@@ -322,3 +322,80 @@ class Stream:
 
     def __del__(self):
         self.close()
+
+
+def _pred_desc(kind, m, dtype, thresholds=(0.0, 0.0), sos_q=1e6):
+    pd = _lib.PredicateDesc()
+    pd.kind = kind
+    pd.codimension = m
+    pd.dtype = dtype
+    pd.sos_q = sos_q
+    pd.thresholds[0] = float(thresholds[0])
+    pd.thresholds[1] = float(thresholds[1]) if len(thresholds) > 1 else 0.0
+    return pd
+
+
+class Batch:
+    """Many meshes of one predicate in one pass (ftk2h_batch_*).  run() takes a list of
+    (mesh, comps) or (mesh, comps, scalar) and returns one Result per job, each equal
+    to extract() on that job alone."""
+
+    def __init__(self, kind: int, m: int, dtype: int, ndims: int, thresholds=(0.0, 0.0), sos_q: float = 1e6,
+                 device: int = 0):
+        self.pd = _pred_desc(kind, m, dtype, thresholds, sos_q)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().ftk2h_batch_create(context(device), C.byref(self.pd), ndims, C.byref(h)))
+        self.h = h.value
+
+    def _jobs(self, jobs):
+        arr = (_lib.Job * len(jobs))()
+        keep = []
+        for i, job in enumerate(jobs):
+            mesh, comps = job[0], list(job[1])
+            scalar = job[2] if len(job) > 2 else None
+            views = (_lib.FieldView * len(comps))(*[field_view(c) for c in comps])
+            keep.append((views, comps, scalar))
+            arr[i].mesh = mesh.desc()
+            arr[i].comps = C.cast(views, C.POINTER(_lib.FieldView))
+            if scalar is not None:
+                sv = _lib.FieldView * 1
+                sva = sv(field_view(scalar))
+                keep.append(sva)
+                arr[i].scalar = C.cast(sva, C.POINTER(_lib.FieldView))
+        return arr, keep
+
+    def set_config(self, grid_per_cu=0, chunk=0, use_graph=True):
+        cfg = _lib.BatchConfig(grid_per_cu, chunk, 1 if use_graph else 0)
+        _lib.check(_lib.lib().ftk2h_batch_set_config(self.h, C.byref(cfg)))
+
+    def config(self):
+        cfg = _lib.BatchConfig()
+        _lib.check(_lib.lib().ftk2h_batch_get_config(self.h, C.byref(cfg)))
+        return {"grid_per_cu": cfg.grid_per_cu, "chunk": cfg.chunk, "use_graph": cfg.use_graph}
+
+    def run(self, jobs, stream=None):
+        arr, keep = self._jobs(jobs)
+        outs = (C.c_void_p * len(jobs))()
+        _lib.check(_lib.lib().ftk2h_batch_run(self.h, len(jobs), arr, outs,
+                                              C.c_void_p(stream) if stream else None))
+        return [Result(outs[i], keep=keep) for i in range(len(jobs))]
+
+    def autotune(self, jobs, reps=2):
+        arr, keep = self._jobs(jobs)
+        cfg = _lib.BatchConfig()
+        ms = C.c_float()
+        _lib.check(_lib.lib().ftk2h_batch_autotune(self.h, len(jobs), arr, reps, C.byref(cfg), C.byref(ms)))
+        return {"grid_per_cu": cfg.grid_per_cu, "chunk": cfg.chunk, "use_graph": cfg.use_graph}, ms.value
+
+    def last_ms(self):
+        buf = (C.c_float * 3)()
+        _lib.check(_lib.lib().ftk2h_batch_last_ms(self.h, buf, 3))
+        return list(buf)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().ftk2h_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -143,6 +143,40 @@ int ftk2h_stream_append(ftk2h_stream* s, const ftk2h_field_view* comps, void* hi
 int ftk2h_stream_finish(ftk2h_stream* s, ftk2h_result** out);
 void ftk2h_stream_close(ftk2h_stream* s);
 
+/* ---- Batching -------------------------------------------------------------------
+ * No counterpart in the reference, which runs one SimplicialEngine::execute()
+ * (engine.hpp:96) per mesh.  Many independent RegularSimplicialMesh jobs of one
+ * predicate (same kind, codimension, dtype and mesh dimension) are extracted in one
+ * pass: one job table on the device, the jobs' tiles scheduled over one persistent
+ * grid, the launch sequence captured in a hipGraph and replayed while its shape is
+ * unchanged.  results[j] equals ftk2h_extract_regular on job j alone, bit for bit.
+ * The results of one run share device memory; each is freed by ftk2h_result_free.
+ * Predicates whose cells are star-fan triangles (codimension d - 2 with d - m = 2)
+ * are not batched (FTK2H_E_UNSUPPORTED). */
+typedef struct ftk2h_batch ftk2h_batch;
+typedef struct {
+  ftk2h_mesh_desc mesh;
+  const ftk2h_field_view* comps;  /* as many views as the predicate reads */
+  const ftk2h_field_view* scalar; /* optional CP scalar field, or NULL */
+} ftk2h_job;
+typedef struct {
+  int grid_per_cu; /* persistent workgroups per compute unit; 0: one workgroup per tile */
+  int chunk;       /* with a persistent grid: 0 = static stride, else tiles per atomic grab */
+  int use_graph;   /* 1: capture the launches in a hipGraph and replay it */
+} ftk2h_batch_config;
+int ftk2h_batch_create(ftk2h_ctx* ctx, const ftk2h_predicate_desc* pred, int ndims, ftk2h_batch** out);
+int ftk2h_batch_set_config(ftk2h_batch* b, const ftk2h_batch_config* cfg);
+int ftk2h_batch_get_config(const ftk2h_batch* b, ftk2h_batch_config* cfg);
+/* hip_stream: NULL for the batch's own stream.  Returns when every result is ready. */
+int ftk2h_batch_run(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** results, void* hip_stream);
+/* Times every candidate schedule on these jobs (reps runs each, best of), keeps the
+ * fastest as the batch's configuration and returns it with its time in *ms. */
+int ftk2h_batch_autotune(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, int reps, ftk2h_batch_config* best,
+                         float* ms);
+/* Device ms of the last run: [0] classify..job bases, [1] emit..link, [2] host wall. */
+int ftk2h_batch_last_ms(const ftk2h_batch* b, float* ms, int n);
+void ftk2h_batch_destroy(ftk2h_batch* b);
+
 const char* ftk2h_last_error(void);
 
 #ifdef __cplusplus
