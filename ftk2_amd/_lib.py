@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftk2h.so")
 
 FTK2H_OK = 0
-STATUS = {0: "OK", 1: "E_INVALID", 2: "E_UNSUPPORTED", 3: "E_OVERFLOW", 4: "E_OOM", 5: "E_HIP"}
+STATUS = {0: "OK", 1: "E_INVALID", 2: "E_UNSUPPORTED", 3: "E_OVERFLOW", 4: "E_OOM", 5: "E_HIP", 6: "E_AGAIN"}
+E_AGAIN = 6
 PRED_CRITICAL_POINT, PRED_CONTOUR, PRED_FIBER = 0, 1, 2
 F32, F64 = 0, 1
 
@@ -86,6 +87,16 @@ SIGNATURES = {
                                        C.POINTER(C.c_float)]),
     "ftk2h_batch_last_ms": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
     "ftk2h_batch_destroy": (None, [_P]),
+    "ftk2h_server_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "ftk2h_server_open": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(PredicateDesc), C.c_int,
+                                    C.c_uint64, C.POINTER(C.c_uint64)]),
+    "ftk2h_server_append": (C.c_int, [_P, C.c_uint64, C.POINTER(FieldView)]),
+    "ftk2h_server_close": (C.c_int, [_P, C.c_uint64]),
+    "ftk2h_server_step": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int, C.POINTER(C.c_int)]),
+    "ftk2h_server_take": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "ftk2h_server_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
+    "ftk2h_server_destroy": (None, [_P]),
 }
 
 _lib = None

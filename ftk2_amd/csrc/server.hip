@@ -1,0 +1,294 @@
+// Serving (ftk2h_server_*, include/ftk2h.h): streaming-tracking requests admitted
+// against a device-memory budget and advanced by continuous batching.  Each
+// request is an ftk2h_stream (execute_stream semantics: last timestep and open
+// union-find frontier stay on the device).  A step takes every admitted stream
+// that has a queued timestep, forms its next slab, and extracts the slabs of one
+// predicate template in one batched pass (ftk2h_batch_run); streams join at the
+// first step after admission and leave when closed and drained.  Because a slab's
+// extraction is independent of the other jobs in its batch, each result equals
+// the one the stream gets alone.
+#include "stream.hpp"
+
+#include <deque>
+#include <map>
+#include <tuple>
+
+using namespace ftk2h;
+using namespace ftk2h_internal;
+
+namespace {
+
+struct QueuedStep {
+  ftk2h_field_view views[3];
+  std::vector<char> host[3];  // owned copies of host views
+};
+
+struct ServedStream {
+  ftk2h_stream* s = nullptr;
+  ftk2h_predicate_desc pred{};
+  int ncomp = 0;
+  int d = 0;
+  uint64_t dims[3] = {1, 1, 1};
+  uint64_t max_t = 0, appended = 0;
+  uint64_t reserved = 0;
+  std::deque<QueuedStep> queue;
+  bool closed = false;
+  bool finished = false;
+  ftk2h_result* result = nullptr;
+};
+
+using BatchKey = std::tuple<int, int, int, int, double, double, double>;
+
+BatchKey key_of(const ftk2h_predicate_desc& p, int d) {
+  return BatchKey(p.kind, d, p.codimension, p.dtype, p.sos_q, p.thresholds[0], p.thresholds[1]);
+}
+
+}  // namespace
+
+struct ftk2h_server {
+  ftk2h_ctx* ctx = nullptr;
+  uint64_t budget = 0, reserved = 0;
+  uint64_t next_id = 1;
+  std::map<uint64_t, ServedStream> streams;
+  std::map<BatchKey, ftk2h_batch*> batches;
+  std::mutex lock;
+};
+
+namespace {
+
+int n_types_of(int d, int m) { return n_types_rt(d, m); }
+
+// Device bytes reserved for a stream: its two resident timesteps, the per-slab
+// work arrays of a batched pass (u64 mask + u32 offset per slab vertex) and an
+// output allowance of 2% of the simplices becoming nodes (key, lambda, parent,
+// track, one edge each), over every slab it may append.
+uint64_t reservation(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int ncomp, uint64_t max_t) {
+  uint64_t nsp = 1;
+  for (int j = 0; j < sd; ++j) nsp *= dims[j];
+  const uint64_t esz = p.dtype == FTK2H_F64 ? 8 : 4;
+  const int d = sd + 1, m = p.codimension;
+  const uint64_t fields = 2 * nsp * esz * (uint64_t)ncomp;
+  const uint64_t work = 2 * nsp * 12;
+  const uint64_t per_node = 8 + (uint64_t)(m + 1) * esz + 4 + 4 + 8;
+  const uint64_t slabs = max_t > 1 ? max_t - 1 : 0;
+  const uint64_t out = slabs * nsp * (uint64_t)n_types_of(d, m) / 50 * per_node;
+  return fields + work + out;
+}
+
+int finish_stream(ServedStream& S) {
+  ftk2h_result* R = nullptr;
+  const int rc = ftk2h_stream_finish(S.s, &R);
+  ftk2h_stream_close(S.s);
+  S.s = nullptr;
+  S.queue.clear();
+  S.finished = true;
+  if (rc) return rc;
+  S.result = R;
+  return FTK2H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftk2h_server_create(ftk2h_ctx* ctx, uint64_t hbm_budget_bytes, ftk2h_server** out) {
+  if (!ctx || !out) return fail(FTK2H_E_INVALID, "null argument");
+  if (hbm_budget_bytes == 0) return fail(FTK2H_E_INVALID, "zero budget");
+  auto s = std::make_unique<ftk2h_server>();
+  s->ctx = ctx;
+  s->budget = hbm_budget_bytes;
+  *out = s.release();
+  return FTK2H_OK;
+}
+
+int ftk2h_server_open(ftk2h_server* s, int spatial_ndims, const uint64_t* spatial_dims,
+                      const ftk2h_predicate_desc* pred, int ncomp, uint64_t max_timesteps, uint64_t* id) {
+  if (!s || !spatial_dims || !pred || !id) return fail(FTK2H_E_INVALID, "null argument");
+  if (spatial_ndims < 1 || spatial_ndims > 3) return fail(FTK2H_E_INVALID, "spatial dims must be 1..3");
+  if (max_timesteps == 0) return fail(FTK2H_E_INVALID, "max_timesteps must be >= 1");
+  std::lock_guard<std::mutex> g(s->lock);
+  const uint64_t need = reservation(spatial_ndims, spatial_dims, *pred, ncomp, max_timesteps);
+  if (need > s->budget) return fail(FTK2H_E_OVERFLOW, "request exceeds the whole device-memory budget");
+  if (s->reserved + need > s->budget) return fail(FTK2H_E_AGAIN, "device-memory budget in use; retry later");
+  ServedStream S;
+  int rc = ftk2h_stream_open(s->ctx, spatial_ndims, spatial_dims, pred, ncomp, &S.s);
+  if (rc) return rc;
+  S.pred = *pred;
+  S.ncomp = ncomp;
+  S.d = spatial_ndims + 1;
+  for (int j = 0; j < spatial_ndims; ++j) S.dims[j] = spatial_dims[j];
+  S.max_t = max_timesteps;
+  S.reserved = need;
+  s->reserved += need;
+  *id = s->next_id++;
+  s->streams.emplace(*id, std::move(S));
+  return FTK2H_OK;
+}
+
+int ftk2h_server_append(ftk2h_server* s, uint64_t id, const ftk2h_field_view* comps) {
+  if (!s || !comps) return fail(FTK2H_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(s->lock);
+  auto it = s->streams.find(id);
+  if (it == s->streams.end()) return fail(FTK2H_E_INVALID, "no such stream");
+  ServedStream& S = it->second;
+  if (S.closed || S.finished) return fail(FTK2H_E_INVALID, "stream is closed");
+  if (S.appended >= S.max_t) return fail(FTK2H_E_OVERFLOW, "more timesteps than the stream was admitted for");
+  QueuedStep q;
+  for (int c = 0; c < S.ncomp; ++c) {
+    if (!comps[c].ptr) return fail(FTK2H_E_INVALID, "null field pointer");
+    q.views[c] = comps[c];
+  }
+  // host views are copied now (the caller may reuse its buffers); device views are
+  // read by the step that consumes them
+  const int sd = S.d - 1;
+  uint64_t nsp = 1;
+  for (int j = 0; j < sd; ++j) nsp *= S.dims[j];
+  const size_t esz = S.pred.dtype == FTK2H_F64 ? 8 : 4;
+  for (int c = 0; c < S.ncomp; ++c) {
+    if (comps[c].on_device) continue;
+    q.host[c].resize(nsp * esz);
+    int64_t lc[3] = {0, 0, 0};
+    const char* src = static_cast<const char*>(comps[c].ptr);
+    for (uint64_t i = 0; i < nsp; ++i) {
+      int64_t idx = 0;
+      for (int j = 0; j < sd; ++j) idx += lc[j] * comps[c].strides[j];
+      std::memcpy(&q.host[c][i * esz], src + idx * (int64_t)esz, esz);
+      for (int j = 0; j < sd; ++j) {
+        if (++lc[j] < (int64_t)S.dims[j]) break;
+        lc[j] = 0;
+      }
+    }
+    q.views[c].ptr = q.host[c].data();
+    int64_t st = 1;
+    for (int j = 0; j < 4; ++j) {
+      q.views[c].strides[j] = j < sd ? st : 0;
+      if (j < sd) st *= (int64_t)S.dims[j];
+    }
+  }
+  S.queue.push_back(std::move(q));
+  S.appended++;
+  return FTK2H_OK;
+}
+
+int ftk2h_server_close(ftk2h_server* s, uint64_t id) {
+  if (!s) return fail(FTK2H_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(s->lock);
+  auto it = s->streams.find(id);
+  if (it == s->streams.end()) return fail(FTK2H_E_INVALID, "no such stream");
+  it->second.closed = true;
+  return FTK2H_OK;
+}
+
+int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done) {
+  if (!s || !n_done || (max_done > 0 && !done)) return fail(FTK2H_E_INVALID, "null argument");
+  *n_done = 0;
+  std::lock_guard<std::mutex> g(s->lock);
+  HIP_TRY(hipSetDevice(s->ctx->device));
+  // 1. every stream takes queued timesteps until its next slab is ready
+  struct Pending {
+    ServedStream* S;
+    ftk2h_job job;
+    ftk2h_field_view views[3];
+  };
+  std::map<BatchKey, std::vector<Pending>> groups;
+  for (auto& [id, S] : s->streams) {
+    if (S.finished) continue;
+    while (!S.queue.empty() && !stream_slab_ready(S.s)) {
+      int rc = stream_put(S.s, S.queue.front().views);
+      if (rc) return rc;
+      S.queue.pop_front();
+    }
+    if (stream_slab_ready(S.s)) {
+      Pending p;
+      p.S = &S;
+      std::memset(&p.job, 0, sizeof(p.job));
+      stream_slab_desc(S.s, &p.job.mesh, p.views);
+      groups[key_of(S.pred, S.d)].push_back(p);
+    }
+  }
+  // the timestep copies were issued on each stream's own HIP stream
+  HIP_TRY(hipDeviceSynchronize());
+  // 2. one batched pass per predicate template
+  for (auto& [key, list] : groups) {
+    ftk2h_batch*& b = s->batches[key];
+    if (!b) {
+      const ServedStream& S0 = *list.front().S;
+      int rc = ftk2h_batch_create(s->ctx, &S0.pred, S0.d, &b);
+      if (rc) return rc;
+      ftk2h_batch_config cfg{4, 4, 0};  // persistent grid, dynamic chunks; shapes change every step
+      if ((rc = ftk2h_batch_set_config(b, &cfg))) return rc;
+    }
+    std::vector<ftk2h_job> jobs(list.size());
+    for (size_t i = 0; i < list.size(); ++i) {
+      jobs[i] = list[i].job;
+      jobs[i].comps = list[i].views;
+      jobs[i].scalar = nullptr;
+    }
+    std::vector<ftk2h_result*> res(list.size(), nullptr);
+    int rc = ftk2h_batch_run(b, (int)jobs.size(), jobs.data(), res.data(), nullptr);
+    if (rc) return rc;
+    for (size_t i = 0; i < list.size(); ++i) {
+      rc = stream_take_slab(list[i].S->s, res[i]);
+      res[i] = nullptr;
+      if (rc) {
+        for (auto* r : res)
+          if (r) ftk2h_result_free(r);
+        return rc;
+      }
+    }
+  }
+  // 3. closed and drained streams leave with their result
+  for (auto& [id, S] : s->streams) {
+    if (S.finished || !S.closed || !S.queue.empty() || stream_slab_ready(S.s)) continue;
+    int rc = finish_stream(S);
+    if (rc) return rc;
+    if (*n_done < max_done) done[*n_done] = id;
+    ++*n_done;
+  }
+  return FTK2H_OK;
+}
+
+int ftk2h_server_take(ftk2h_server* s, uint64_t id, ftk2h_result** out) {
+  if (!s || !out) return fail(FTK2H_E_INVALID, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> g(s->lock);
+  auto it = s->streams.find(id);
+  if (it == s->streams.end()) return fail(FTK2H_E_INVALID, "no such stream");
+  if (!it->second.finished) return fail(FTK2H_E_AGAIN, "stream not finished");
+  *out = it->second.result;
+  s->reserved -= it->second.reserved;
+  s->streams.erase(it);
+  return FTK2H_OK;
+}
+
+int ftk2h_server_stats(const ftk2h_server* s, uint64_t* reserved_bytes, uint64_t* device_bytes, int* active,
+                       int* queued_timesteps) {
+  if (!s) return fail(FTK2H_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(const_cast<ftk2h_server*>(s)->lock);
+  uint64_t dev = 0;
+  int act = 0, q = 0;
+  for (const auto& [id, S] : s->streams) {
+    if (!S.finished) {
+      ++act;
+      dev += stream_device_bytes(S.s);
+    }
+    q += (int)S.queue.size();
+  }
+  if (reserved_bytes) *reserved_bytes = s->reserved;
+  if (device_bytes) *device_bytes = dev;
+  if (active) *active = act;
+  if (queued_timesteps) *queued_timesteps = q;
+  return FTK2H_OK;
+}
+
+void ftk2h_server_destroy(ftk2h_server* s) {
+  if (!s) return;
+  for (auto& [id, S] : s->streams) {
+    if (S.s) ftk2h_stream_close(S.s);
+    if (S.result) ftk2h_result_free(S.result);
+  }
+  for (auto& [k, b] : s->batches) ftk2h_batch_destroy(b);
+  delete s;
+}
+
+}  // extern "C"

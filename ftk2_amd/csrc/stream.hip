@@ -11,7 +11,7 @@
 //     the track components).
 // The result of finish() equals the one-shot extraction over all timesteps (the
 // reference's contract, test_examples.cpp:259-297), which the tests check.
-#include "internal.hpp"
+#include "stream.hpp"
 
 using namespace ftk2h;
 using namespace ftk2h_internal;
@@ -118,6 +118,7 @@ struct ftk2h_stream {
   hipStream_t st = nullptr;
   void* fields[3] = {nullptr, nullptr, nullptr};  // 2 * nsp values each (ping-pong)
   uint64_t t = 0;                                 // timesteps appended
+  uint64_t slabs = 0;                             // slabs extracted
   ftk2h_result* open = nullptr;                   // previous slab, not committed yet
   uint32_t open_nlow = 0;
   uint32_t committed = 0;                         // committed nodes
@@ -193,6 +194,112 @@ int commit_open(ftk2h_stream* S, const ftk2h_result* next, uint32_t next_nlow) {
 
 }  // namespace
 
+namespace ftk2h_internal {
+
+int stream_put(ftk2h_stream* S, const ftk2h_field_view* comps) {
+  if (stream_slab_ready(S)) return fail(FTK2H_E_INVALID, "stream: the pending slab must be extracted first");
+  const uint64_t half = S->t % 2;
+  // copy the timestep into its ping-pong half (x fastest, contiguous)
+  for (int c = 0; c < S->ncomp; ++c) {
+    char* dst = static_cast<char*>(S->fields[c]) + half * S->nsp * S->esz;
+    const ftk2h_field_view& v = comps[c];
+    if (!v.ptr) return fail(FTK2H_E_INVALID, "null field pointer");
+    int64_t want = 1;
+    bool contiguous = true;
+    for (int j = 0; j < S->sd; ++j) {
+      contiguous = contiguous && (S->spatial[j] == 1 || v.strides[j] == want);
+      want *= (int64_t)S->spatial[j];
+    }
+    if (contiguous) {
+      HIP_TRY(hipMemcpyAsync(dst, v.ptr, S->nsp * S->esz, v.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             S->st));
+      if (!v.on_device) HIP_TRY(hipStreamSynchronize(S->st));  // the caller may free its buffer
+    } else {
+      if (v.on_device) return fail(FTK2H_E_UNSUPPORTED, "stream: device views must be contiguous");
+      std::vector<char> pack(S->nsp * S->esz);
+      int64_t lc[3] = {0, 0, 0};
+      for (uint64_t i = 0; i < S->nsp; ++i) {
+        int64_t idx = 0;
+        for (int j = 0; j < S->sd; ++j) idx += lc[j] * v.strides[j];
+        std::memcpy(&pack[i * S->esz], static_cast<const char*>(v.ptr) + idx * (int64_t)S->esz, S->esz);
+        for (int j = 0; j < S->sd; ++j) {
+          if (++lc[j] < (int64_t)S->spatial[j]) break;
+          lc[j] = 0;
+        }
+      }
+      HIP_TRY(hipMemcpyAsync(dst, pack.data(), pack.size(), hipMemcpyHostToDevice, S->st));
+      HIP_TRY(hipStreamSynchronize(S->st));
+    }
+  }
+  S->t++;
+  return FTK2H_OK;
+}
+
+bool stream_slab_ready(const ftk2h_stream* S) { return S->t >= 2 && S->slabs + 1 < S->t; }
+
+// Slab [t-2, t-1]: views over both halves (time stride = distance between halves),
+// offset t-2 in time, global time extent t.
+void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_view* views) {
+  const uint64_t slab_t = S->t - 2;
+  const uint64_t hp = slab_t % 2;
+  std::memset(md, 0, sizeof(*md));
+  md->ndims = S->sd + 1;
+  for (int j = 0; j < 4; ++j) {
+    md->local_dims[j] = j < S->sd ? S->spatial[j] : 1;
+    md->offset[j] = 0;
+    md->global_dims[j] = j < S->sd ? S->spatial[j] : 1;
+  }
+  md->local_dims[S->sd] = 2;
+  md->offset[S->sd] = slab_t;
+  md->global_dims[S->sd] = slab_t + 2;
+  for (int c = 0; c < S->ncomp; ++c) {
+    views[c] = ftk2h_field_view{};
+    views[c].on_device = 1;
+    views[c].ptr = static_cast<char*>(S->fields[c]) + hp * S->nsp * S->esz;
+    int64_t s = 1;
+    for (int j = 0; j < S->sd; ++j) {
+      views[c].strides[j] = s;
+      s *= (int64_t)S->spatial[j];
+    }
+    views[c].strides[S->sd] = hp == 0 ? (int64_t)S->nsp : -(int64_t)S->nsp;
+  }
+}
+
+// R = extraction of the pending slab (ownership passes to the stream): commit the
+// previously open slab against it and keep R open.
+int stream_take_slab(ftk2h_stream* S, ftk2h_result* R) {
+  const uint64_t slab_t = S->t - 2;
+  uint32_t nlow = 0;
+  int rc = slab_lower_count(S, R, slab_t, &nlow);
+  if (rc) {
+    ftk2h_result_free(R);
+    return rc;
+  }
+  if (S->open) {
+    rc = commit_open(S, R, nlow);
+    ftk2h_result_free(S->open);
+    S->open = nullptr;
+    if (rc) {
+      ftk2h_result_free(R);
+      return rc;
+    }
+  }
+  S->open = R;
+  S->open_nlow = nlow;
+  S->slabs++;
+  return FTK2H_OK;
+}
+
+uint64_t stream_timesteps(const ftk2h_stream* S) { return S->t; }
+
+uint64_t stream_device_bytes(const ftk2h_stream* S) {
+  uint64_t b = 2 * S->nsp * S->esz * (uint64_t)S->ncomp;
+  b += S->keys.cap + S->lambda.cap + S->parent.cap + S->cells.cap;
+  return b;
+}
+
+}  // namespace ftk2h_internal
+
 extern "C" {
 
 int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial_dims,
@@ -237,93 +344,25 @@ int ftk2h_stream_append(ftk2h_stream* S, const ftk2h_field_view* comps, void* hi
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, S->st));
-  const uint64_t half = S->t % 2;
-  // copy the timestep into its ping-pong half (x fastest, contiguous)
-  for (int c = 0; c < S->ncomp; ++c) {
-    char* dst = static_cast<char*>(S->fields[c]) + half * S->nsp * S->esz;
-    const ftk2h_field_view& v = comps[c];
-    int64_t want = 1;
-    bool contiguous = true;
-    for (int j = 0; j < S->sd; ++j) {
-      contiguous = contiguous && (S->spatial[j] == 1 || v.strides[j] == want);
-      want *= (int64_t)S->spatial[j];
-    }
-    if (contiguous) {
-      HIP_TRY(hipMemcpyAsync(dst, v.ptr, S->nsp * S->esz, v.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                             S->st));
-    } else {
-      if (v.on_device) return fail(FTK2H_E_UNSUPPORTED, "stream: device views must be contiguous");
-      std::vector<char> pack(S->nsp * S->esz);
-      int64_t lc[3] = {0, 0, 0};
-      for (uint64_t i = 0; i < S->nsp; ++i) {
-        int64_t idx = 0;
-        for (int j = 0; j < S->sd; ++j) idx += lc[j] * v.strides[j];
-        std::memcpy(&pack[i * S->esz], static_cast<const char*>(v.ptr) + idx * (int64_t)S->esz, S->esz);
-        for (int j = 0; j < S->sd; ++j) {
-          if (++lc[j] < (int64_t)S->spatial[j]) break;
-          lc[j] = 0;
-        }
-      }
-      HIP_TRY(hipMemcpyAsync(dst, pack.data(), pack.size(), hipMemcpyHostToDevice, S->st));
-      HIP_TRY(hipStreamSynchronize(S->st));
-    }
-  }
-  S->t++;
-  if (S->t >= 2) {
-    // slab [t-2, t-1]: views over both halves (time stride = distance between halves)
-    const uint64_t slab_t = S->t - 2;
-    const uint64_t hp = slab_t % 2;
-    ftk2h_mesh_desc md{};
-    md.ndims = S->sd + 1;
-    for (int j = 0; j < 4; ++j) {
-      md.local_dims[j] = j < S->sd ? S->spatial[j] : 1;
-      md.offset[j] = 0;
-      md.global_dims[j] = j < S->sd ? S->spatial[j] : 1;
-    }
-    md.local_dims[S->sd] = 2;
-    md.offset[S->sd] = slab_t;
-    md.global_dims[S->sd] = slab_t + 2;
+  int rc = stream_put(S, comps);
+  if (!rc && stream_slab_ready(S)) {
+    ftk2h_mesh_desc md;
     ftk2h_field_view views[3];
-    for (int c = 0; c < S->ncomp; ++c) {
-      views[c] = ftk2h_field_view{};
-      views[c].on_device = 1;
-      views[c].ptr = static_cast<char*>(S->fields[c]) + hp * S->nsp * S->esz;
-      int64_t s = 1;
-      for (int j = 0; j < S->sd; ++j) {
-        views[c].strides[j] = s;
-        s *= (int64_t)S->spatial[j];
-      }
-      views[c].strides[S->sd] = hp == 0 ? (int64_t)S->nsp : -(int64_t)S->nsp;
-    }
-    HIP_TRY(hipStreamSynchronize(S->st));
+    stream_slab_desc(S, &md, views);
     ftk2h_result* R = nullptr;
-    int rc = ftk2h_extract_regular(S->ctx, &md, &S->pred, views, S->ncomp, nullptr, nullptr, &R, S->st);
-    if (rc) return rc;
-    uint32_t nlow = 0;
-    if ((rc = slab_lower_count(S, R, slab_t, &nlow))) {
-      ftk2h_result_free(R);
-      return rc;
-    }
-    if (S->open) {
-      rc = commit_open(S, R, nlow);
-      ftk2h_result_free(S->open);
-      S->open = nullptr;
-      if (rc) {
-        ftk2h_result_free(R);
-        return rc;
-      }
-    }
-    S->open = R;
-    S->open_nlow = nlow;
+    rc = ftk2h_extract_regular(S->ctx, &md, &S->pred, views, S->ncomp, nullptr, nullptr, &R, S->st);
+    if (!rc) rc = stream_take_slab(S, R);
   }
-  HIP_TRY(hipEventRecord(e1, S->st));
-  HIP_TRY(hipEventSynchronize(e1));
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  S->ms_total += ms;
+  if (!rc) {
+    HIP_TRY(hipEventRecord(e1, S->st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    S->ms_total += ms;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return FTK2H_OK;
+  return rc;
 }
 
 int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {

@@ -1,0 +1,20 @@
+// Steps of a streaming extraction (ftk2h_stream_*), callable one by one so that the
+// server can run the pending slabs of many streams as one batch.
+#pragma once
+#include "internal.hpp"
+
+namespace ftk2h_internal {
+
+// Copy the next timestep into its device half.  Fails if a slab is pending.
+int stream_put(ftk2h_stream* S, const ftk2h_field_view* comps);
+// A slab [t-2, t-1] awaits extraction.
+bool stream_slab_ready(const ftk2h_stream* S);
+// Mesh and device views of the pending slab (views[ncomp]).
+void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_view* views);
+// Hand the pending slab's extraction to the stream (takes ownership of R).
+int stream_take_slab(ftk2h_stream* S, ftk2h_result* R);
+uint64_t stream_timesteps(const ftk2h_stream* S);
+// Device memory held by the stream (timestep halves + committed nodes/cells).
+uint64_t stream_device_bytes(const ftk2h_stream* S);
+
+}  // namespace ftk2h_internal

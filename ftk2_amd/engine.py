@@ -399,3 +399,67 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+class Server:
+    """Continuous-batching request server (ftk2h_server_*): streams admitted against a
+    device-memory budget; step() advances every admitted stream by one slab, the
+    slabs of one predicate template in one batched pass."""
+
+    def __init__(self, budget_bytes: int, device: int = 0):
+        h = C.c_void_p()
+        _lib.check(_lib.lib().ftk2h_server_create(context(device), int(budget_bytes), C.byref(h)))
+        self.h = h.value
+        self._keep = {}
+
+    def open(self, spatial_dims, kind: int, m: int, dtype: int, ncomp: int, max_timesteps: int,
+             thresholds=(0.0, 0.0), sos_q: float = 1e6):
+        """Stream id, or None when the budget is in use (retry after taking results)."""
+        pd = _pred_desc(kind, m, dtype, thresholds, sos_q)
+        sp = [int(x) for x in spatial_dims]
+        dims = (C.c_uint64 * 3)(*(sp + [1] * (3 - len(sp))))
+        sid = C.c_uint64()
+        rc = _lib.lib().ftk2h_server_open(self.h, len(sp), dims, C.byref(pd), ncomp, int(max_timesteps),
+                                          C.byref(sid))
+        if rc == _lib.E_AGAIN:
+            return None
+        _lib.check(rc)
+        self._keep[sid.value] = []
+        return sid.value
+
+    def append(self, sid: int, comps):
+        comps = list(comps)
+        views = (_lib.FieldView * len(comps))(*[field_view(c) for c in comps])
+        _lib.check(_lib.lib().ftk2h_server_append(self.h, sid, views))
+        self._keep[sid].append(comps)  # device views are read by a later step
+
+    def close_stream(self, sid: int):
+        _lib.check(_lib.lib().ftk2h_server_close(self.h, sid))
+
+    def step(self, max_done: int = 4096):
+        done = (C.c_uint64 * max_done)()
+        n = C.c_int()
+        _lib.check(_lib.lib().ftk2h_server_step(self.h, done, max_done, C.byref(n)))
+        return [done[i] for i in range(min(n.value, max_done))]
+
+    def take(self, sid: int) -> Result:
+        out = C.c_void_p()
+        _lib.check(_lib.lib().ftk2h_server_take(self.h, sid, C.byref(out)))
+        self._keep.pop(sid, None)
+        return Result(out.value)
+
+    def stats(self):
+        r = C.c_uint64()
+        d = C.c_uint64()
+        a = C.c_int()
+        q = C.c_int()
+        _lib.check(_lib.lib().ftk2h_server_stats(self.h, C.byref(r), C.byref(d), C.byref(a), C.byref(q)))
+        return {"reserved_bytes": r.value, "device_bytes": d.value, "active": a.value, "queued": q.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().ftk2h_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -65,3 +65,39 @@ def scalar_config(n=384, nt=4, ngauss=64, seed=0, dtype=np.float32):
     nn = (n, n, n) if np.isscalar(n) else tuple(n)
     return gaussians(nn, nt, ngauss, sigma=(8.0, 24.0), seed=seed, gradient=False,
                      subtract_mean=True, dtype=dtype)[0]
+
+
+def gaussian_gradient_device(spatial, nt, ngauss, sigma=(4.0, 10.0), amp=(0.5, 1.0), vel=1.0, seed=0,
+                             device="cuda"):
+    """Analytic gradient of a sum of moving Gaussians evaluated on the GPU (torch,
+    FP64): the same distributions as gaussians() (amplitudes +-U[amp], sigma
+    U[sigma], centres uniform in the domain, velocity U[-vel, vel] cells/step), for
+    request pools of the serving benchmark.  Returns len(spatial) tensors shaped
+    (nt, [nz,] ny, nx), each timestep a contiguous x-fastest block."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sd = len(spatial)
+    axes = [torch.arange(n, dtype=torch.float64, device=device) for n in spatial]
+    shape = tuple(reversed(spatial))
+    out = [torch.zeros((nt,) + shape, dtype=torch.float64, device=device) for _ in range(sd)]
+    A = rng.uniform(*amp, size=ngauss) * np.where(rng.random(ngauss) < 0.5, -1.0, 1.0)
+    S = rng.uniform(*sigma, size=ngauss)
+    C0 = np.stack([rng.uniform(0, n - 1, size=ngauss) for n in spatial], axis=1)
+    V = rng.uniform(-vel, vel, size=(ngauss, sd))
+    for t in range(nt):
+        for g in range(ngauss):
+            c = C0[g] + V[g] * t
+            s2 = S[g] * S[g]
+            d = [axes[a] - c[a] for a in range(sd)]
+            e = [torch.exp(-(d[a] * d[a]) / (2 * s2)) for a in range(sd)]
+            # separable: g(x) = prod_a e_a(x_a); broadcast with x the last (fastest) axis
+            def bc(v, a):
+                view = [1] * sd
+                view[sd - 1 - a] = -1
+                return v.view(view)
+            gval = bc(e[0], 0)
+            for a in range(1, sd):
+                gval = gval * bc(e[a], a)
+            for a in range(sd):
+                out[a][t] += (-A[g] / s2) * bc(d[a], a) * gval
+    return out

@@ -40,7 +40,8 @@ typedef enum {
   FTK2H_E_UNSUPPORTED = 2, /* (predicate, ndims, codimension, dtype) not built */
   FTK2H_E_OVERFLOW = 3,    /* a fixed capacity in ftk2h_options was too small */
   FTK2H_E_OOM = 4,         /* device allocation failed */
-  FTK2H_E_HIP = 5          /* a HIP runtime call failed */
+  FTK2H_E_HIP = 5,         /* a HIP runtime call failed */
+  FTK2H_E_AGAIN = 6        /* not now: device-memory budget in use or result not ready */
 } ftk2h_status;
 
 typedef enum {
@@ -176,6 +177,32 @@ int ftk2h_batch_autotune(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, int r
 /* Device ms of the last run: [0] classify..job bases, [1] emit..link, [2] host wall. */
 int ftk2h_batch_last_ms(const ftk2h_batch* b, float* ms, int n);
 void ftk2h_batch_destroy(ftk2h_batch* b);
+
+/* ---- Serving ----------------------------------------------------------------------
+ * No counterpart in the reference.  Requests are streams (execute_stream semantics,
+ * engine.hpp:214-270) admitted against a device-memory budget.  Each
+ * ftk2h_server_step extracts the next slab of every admitted stream that has a
+ * queued timestep, the slabs of one predicate template as one ftk2h_batch_run
+ * (continuous batching: streams join at the first step after admission and leave
+ * when closed and drained).  A stream's result equals the one it gets alone. */
+typedef struct ftk2h_server ftk2h_server;
+int ftk2h_server_create(ftk2h_ctx* ctx, uint64_t hbm_budget_bytes, ftk2h_server** out);
+/* Admit a stream of at most max_timesteps timesteps.  FTK2H_E_AGAIN: its device
+ * reservation does not fit the budget now (retry after results are taken);
+ * FTK2H_E_OVERFLOW: it can never fit. */
+int ftk2h_server_open(ftk2h_server* s, int spatial_ndims, const uint64_t* spatial_dims,
+                      const ftk2h_predicate_desc* pred, int ncomp, uint64_t max_timesteps, uint64_t* id);
+/* Queue one timestep (views indexed by spatial coordinates).  Host views are copied
+ * at once; device views are read by the step that consumes them. */
+int ftk2h_server_append(ftk2h_server* s, uint64_t id, const ftk2h_field_view* comps);
+int ftk2h_server_close(ftk2h_server* s, uint64_t id); /* no more timesteps */
+/* One continuous-batching step; done[0..min(n_done, max_done)) = streams finished in it. */
+int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done);
+/* Result of a finished stream (FTK2H_E_AGAIN before); releases its reservation. */
+int ftk2h_server_take(ftk2h_server* s, uint64_t id, ftk2h_result** out);
+int ftk2h_server_stats(const ftk2h_server* s, uint64_t* reserved_bytes, uint64_t* device_bytes, int* active,
+                       int* queued_timesteps);
+void ftk2h_server_destroy(ftk2h_server* s);
 
 const char* ftk2h_last_error(void);
 

@@ -1,0 +1,139 @@
+"""Serving mode (ftk2h_server_* + ftk2_amd/serve.py): requests of mixed sizes,
+timestep counts and templates served together by continuous batching; each result
+equals the same request processed alone (ftk2h_stream, one slab per append), bit for
+bit, also when the device-memory budget makes requests wait for admission."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from ftk2_amd import engine, serve, synth
+from ftk2_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same(a, b, what):
+    assert np.array_equal(a.vertices, b.vertices), f"{what}: node simplices differ"
+    assert np.array_equal(a.lam, b.lam), f"{what}: lambda differs"
+    assert np.array_equal(a.bary, b.bary), f"{what}: barycentrics differ"
+    assert np.array_equal(a.track_id, b.track_id), f"{what}: track ids differ"
+    assert sorted(a.connectivity) == sorted(b.connectivity), f"{what}: connectivity dims differ"
+    for d in a.connectivity:
+        assert np.array_equal(a.connectivity[d], b.connectivity[d]), f"{what}: connectivity differs"
+
+
+def mixed_requests(n, seed, device=False, arrival_rate=None):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    arr = serve.poisson_arrivals(n, arrival_rate, rng) if arrival_rate else np.zeros(n)
+    for i in range(n):
+        T = int(rng.integers(1, 7))
+        if rng.random() < 0.7:
+            side = int(rng.integers(8, 40))
+            dt = _lib.F64 if rng.random() < 0.8 else _lib.F32
+            comps = synth.gaussians((side, side + 3), T, 6, seed=seed * 100 + i,
+                                    dtype=np.float64 if dt == _lib.F64 else np.float32)
+            spatial, m = (side, side + 3), 2
+        else:
+            side = int(rng.integers(5, 10))
+            comps = synth.cp3d_config(n=side, nt=T, ngauss=4, seed=seed * 100 + i, zero_frac=0.02,
+                                      dup_frac=0.02)
+            spatial, m, dt = (side, side, side), 3, _lib.F64
+        steps = []
+        for t in range(T):
+            sl = [c[..., t] for c in comps]
+            if device:
+                import torch
+                sl = [torch.from_numpy(np.ascontiguousarray(s.transpose())).cuda().permute(
+                    *reversed(range(s.ndim))) for s in sl]
+            steps.append(sl)
+        hc = int(np.prod([s - 1 for s in spatial])) * max(0, T - 1)
+        reqs.append(serve.Request(i, float(arr[i]), spatial, _lib.PRED_CRITICAL_POINT, m, dt, steps,
+                                  simplices=hc * (12 if m == 2 else 60), meta={"comps": comps}))
+    return reqs
+
+
+def alone(rq):
+    st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]))
+    for comps in rq.timesteps:
+        st.append(comps)
+    res = st.finish().to_complex()
+    st.close()
+    return res
+
+
+@pytest.mark.parametrize("device", [False, True])
+def test_served_equals_alone(device):
+    reqs = mixed_requests(14, seed=1, device=device)
+    srv = engine.Server(64 << 30)
+    outs, wall, rej = serve.run(srv, reqs, keep_results=True)
+    assert rej == []
+    assert len(outs) == len(reqs)
+    for o in outs:
+        rq = reqs[o.rid]
+        say(f"[serve] request {o.rid}: spatial={rq.spatial} T={len(rq.timesteps)} m={rq.m}")
+        same(o.result.to_complex(), alone(rq), f"request {o.rid}")
+    assert srv.stats()["reserved_bytes"] == 0
+
+
+def test_budget_makes_requests_wait_results_unchanged():
+    reqs = mixed_requests(10, seed=2, arrival_rate=1000.0)
+    # room for about two of the larger requests at a time
+    srv = engine.Server(600 << 10)
+    outs, wall, rej = serve.run(srv, reqs, keep_results=True)
+    assert len(outs) + len(rej) == len(reqs)
+    assert len(outs) >= 8
+    waited = sum(1 for o in outs if o.admitted > o.arrival + 1e-4)
+    say(f"[serve] {waited} of {len(outs)} requests waited for admission; rejected {rej}")
+    for o in outs:
+        same(o.result.to_complex(), alone(reqs[o.rid]), f"request {o.rid}")
+
+
+def test_served_equals_oneshot_and_oracle():
+    reqs = mixed_requests(4, seed=3)
+    srv = engine.Server(64 << 30)
+    outs, _, _ = serve.run(srv, reqs, keep_results=True)
+    for o in outs:
+        rq = reqs[o.rid]
+        comps = rq.meta["comps"]
+        if len(rq.timesteps) < 2:
+            assert len(o.result.to_complex().vertices) == 0  # execute_stream: fewer than 2 timesteps
+            continue
+        cx = o.result.to_complex()
+        mesh = engine.RegularSimplicialMesh(list(comps[0].shape))
+        one = engine.extract(mesh, rq.kind, rq.m, comps).to_complex()
+        same(cx, one, f"request {o.rid} vs one-shot")
+        ref = oracle.run(comps, oracle.PRED_CP, rq.m)
+        assert np.array_equal(cx.vertices, ref["verts"])
+        assert np.array_equal(cx.track_id, ref["track"])
+
+
+def test_server_rejects_and_reports():
+    srv = engine.Server(1 << 20)
+    with pytest.raises(_lib.Ftk2hError) as e:
+        srv.open((512, 512), _lib.PRED_CRITICAL_POINT, 2, _lib.F64, 2, 8)  # never fits
+    assert e.value.code == 3
+    sid = srv.open((8, 8), _lib.PRED_CRITICAL_POINT, 2, _lib.F64, 2, 3)
+    with pytest.raises(_lib.Ftk2hError) as e:
+        srv.take(sid)  # not finished
+    assert e.value.code == _lib.E_AGAIN
+    u, v = synth.cp2d_config(nx=8, ny=8, nt=3)
+    for t in range(3):
+        srv.append(sid, [u[:, :, t], v[:, :, t]])
+    with pytest.raises(_lib.Ftk2hError):
+        srv.append(sid, [u[:, :, 0], v[:, :, 0]])  # beyond max_timesteps
+    srv.close_stream(sid)
+    done = []
+    while not done:
+        done = srv.step()
+    assert done == [sid]
+    r = srv.take(sid).to_complex()
+    ref = oracle.run([u, v], oracle.PRED_CP, 2)
+    assert np.array_equal(r.vertices, ref["verts"])
