@@ -59,6 +59,161 @@ def cpu_baseline(comps, n_sample):
                       f"({n_simp} tets, {len(r['verts'])} nodes, {dt:.2f} s)"}
 
 
+def batch_leg(seed, njobs, reps, log_every=256):
+    """1024 independent CriticalPointPredicate<2,double> jobs (spatial sides log-uniform
+    in [32, 1024], timesteps uniform in [2, 16], 4-64 moving Gaussians): per-job plain
+    extraction vs one batched pass with the autotuned schedule."""
+    import torch
+    from ftk2_amd import _lib, engine, synth
+    rng = np.random.default_rng(7919 + seed)
+    jobs, simp, in_bytes = [], 0, 0
+    t0 = time.perf_counter()
+    for i in range(njobs):
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(32), np.log(1024))))) for _ in range(2))
+        nt = int(rng.integers(2, 17))
+        ng = int(rng.integers(4, 65))
+        u, v = synth.gaussian_gradient_device((nx, ny), nt, ng, sigma=(4.0, 10.0), seed=seed * 100003 + i)
+        # index [x, y, t] views of (t, y, x) blocks
+        jobs.append((engine.RegularSimplicialMesh([nx, ny, nt]), [u.permute(2, 1, 0), v.permute(2, 1, 0)]))
+        simp += (nx - 1) * (ny - 1) * (nt - 1) * 12
+        in_bytes += 2 * nx * ny * nt * 8
+    torch.cuda.synchronize()
+    log(f"[batch] {njobs} jobs generated on the GPU in {time.perf_counter() - t0:.1f} s: {simp} triangles, "
+        f"{in_bytes / 1e9:.2f} GB of fields")
+    # plain: one ftk2h_extract_regular per job, sequentially
+    engine.extract(jobs[0][0], _lib.PRED_CRITICAL_POINT, 2, jobs[0][1])  # warm
+    t0 = time.perf_counter()
+    n_nodes = 0
+    for i, (mesh, comps) in enumerate(jobs):
+        r = engine.extract(mesh, _lib.PRED_CRITICAL_POINT, 2, comps)
+        n_nodes += r.sizes()[0]
+        del r
+        if (i + 1) % log_every == 0:
+            log(f"[batch] plain {i + 1}/{njobs}")
+    plain_s = time.perf_counter() - t0
+    b = engine.Batch(_lib.PRED_CRITICAL_POINT, 2, _lib.F64, 3)
+    t0 = time.perf_counter()
+    cfg, tuned_ms = b.autotune(jobs, reps=1)
+    log(f"[batch] autotune: {cfg} at {tuned_ms:.2f} ms ({time.perf_counter() - t0:.1f} s)")
+    times = []
+    for r_ in range(reps):
+        t0 = time.perf_counter()
+        res = b.run(jobs)
+        times.append(time.perf_counter() - t0)
+        if r_ < reps - 1:
+            del res
+    dev_ms = b.last_ms()
+    # equal results: a sample of jobs against their plain extraction
+    sample = rng.choice(njobs, size=min(8, njobs), replace=False)
+    exact = True
+    for i in sample:
+        a = res[i].to_complex()
+        p = engine.extract(jobs[i][0], _lib.PRED_CRITICAL_POINT, 2, jobs[i][1]).to_complex()
+        exact = exact and np.array_equal(a.vertices, p.vertices) and np.array_equal(a.lam, p.lam) and \
+            np.array_equal(a.track_id, p.track_id) and all(
+                np.array_equal(a.connectivity[d], p.connectivity[d]) for d in p.connectivity)
+    batch_nodes = sum(r.sizes()[0] for r in res)
+    del res
+    b.close()
+    t_b = float(np.mean(times))
+    return {"workload": "1024 x CriticalPointPredicate<2,double>, sides log-U[32,1024], T U[2,16], 4-64 Gaussians",
+            "jobs": njobs, "simplices": int(simp), "input_bytes": int(in_bytes), "nodes": int(batch_nodes),
+            "plain": {"simplices_per_s": simp / plain_s, "ms": plain_s * 1e3, "nodes": int(n_nodes)},
+            "batched": {"simplices_per_s": simp / t_b, "ms": t_b * 1e3, "ms_runs": [x * 1e3 for x in times],
+                        "device_ms": {"classify_to_bases": dev_ms[0], "emit_to_end": dev_ms[1]},
+                        "config": cfg, "autotune_ms": tuned_ms},
+            "speedup": plain_s / t_b, "bit_exact_sample": bool(exact) and batch_nodes == n_nodes}
+
+
+def serve_leg(seed, n_req, rates_x, budget_gb, plain_sample):
+    """Serving: Poisson arrivals of streaming-tracking requests, 70% 2D FP64 CP at
+    128^2-2048^2 and 30% 3D FP64 CP at 64^3-256^3, 1-32 timesteps each, admitted
+    against a device-memory budget and continuously batched.  Offered rate swept in
+    multiples of the plain (one request at a time) service rate."""
+    import torch
+    from ftk2_amd import _lib, engine, serve, synth
+    rng = np.random.default_rng(104729 + seed)
+    sides2, sides3, tmax = [128, 256, 512, 1024, 2048], [64, 128, 256], 32
+    t0 = time.perf_counter()
+    pools = {}
+    for n in sides2:
+        pools[(2, n)] = synth.gaussian_gradient_device((n, n), tmax, 16, sigma=(4.0, 10.0), seed=seed * 7 + n)
+    for n in sides3:
+        pools[(3, n)] = synth.gaussian_gradient_device((n, n, n), tmax, 32, sigma=(8.0, 24.0), seed=seed * 11 + n)
+    torch.cuda.synchronize()
+    log(f"[serve] request pools generated in {time.perf_counter() - t0:.1f} s")
+
+    def make(nreq, rate, rs):
+        arr = serve.poisson_arrivals(nreq, rate, rs) if rate else np.zeros(nreq)
+        out = []
+        for i in range(nreq):
+            if rs.random() < 0.7:
+                sd, n = 2, int(sides2[rs.integers(len(sides2))])
+            else:
+                sd, n = 3, int(sides3[rs.integers(len(sides3))])
+            T = int(rs.integers(1, tmax + 1))
+            pool = pools[(sd, n)]
+            perm = tuple(reversed(range(sd)))
+            steps = [[p[t].permute(*perm) for p in pool] for t in range(T)]
+            simp = (n - 1) ** sd * max(0, T - 1) * (12 if sd == 2 else 60)
+            out.append(serve.Request(i, float(arr[i]), (n,) * sd, _lib.PRED_CRITICAL_POINT, sd, _lib.F64, steps,
+                                     simplices=simp, meta={"sd": sd, "n": n, "T": T}))
+        return out
+
+    # plain: the same kind of requests one at a time through ftk2h_stream
+    plain_reqs = make(plain_sample, None, np.random.default_rng(seed + 1))
+    t0 = time.perf_counter()
+    for rq in plain_reqs:
+        st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]))
+        for comps in rq.timesteps:
+            st.append(comps)
+        r = st.finish()
+        del r
+        st.close()
+    plain_s = time.perf_counter() - t0
+    plain_work = sum(r.simplices for r in plain_reqs)
+    plain_rate = plain_sample / plain_s  # requests/s one at a time
+    log(f"[serve] plain: {plain_sample} requests in {plain_s:.2f} s ({plain_rate:.1f} req/s)")
+    points = []
+    for k, x in enumerate(rates_x):
+        rate = x * plain_rate
+        reqs = make(n_req, rate, np.random.default_rng(seed * 1000 + k))
+        srv = engine.Server(int(budget_gb * (1 << 30)))
+        outs, wall, rej = serve.run(srv, reqs)
+        pt = serve.summarize(outs, reqs, wall)
+        pt.update({"offered_req_per_s": rate, "offered_x_plain": x, "rejected": len(rej)})
+        points.append(pt)
+        log(f"[serve] offered {rate:.1f} req/s ({x}x plain): p50 {pt['p50_ms']:.1f} ms p99 {pt['p99_ms']:.1f} ms "
+            f"{pt['throughput_simplices_per_s']:.3e} simplices/s")
+        srv.close()
+    # each result the same alone and in the mix: a few requests served together vs alone
+    check = make(6, None, np.random.default_rng(seed + 99))
+    check = [r for r in check if r.meta["n"] <= 512] or check[:2]
+    srv = engine.Server(int(budget_gb * (1 << 30)))
+    outs, _, _ = serve.run(srv, check, keep_results=True)
+    exact = len(outs) == len(check)
+    by_rid = {r.rid: r for r in check}
+    for o in outs:
+        rq = by_rid[o.rid]
+        st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]))
+        for comps in rq.timesteps:
+            st.append(comps)
+        a, b = o.result.to_complex(), st.finish().to_complex()
+        st.close()
+        exact = exact and np.array_equal(a.vertices, b.vertices) and np.array_equal(a.lam, b.lam) and \
+            np.array_equal(a.track_id, b.track_id)
+    srv.close()
+    sat = max(points, key=lambda p: p["throughput_simplices_per_s"])
+    plain_tp = plain_work / plain_s
+    return {"workload": "Poisson streams: 70% 2D FP64 CP 128^2-2048^2, 30% 3D FP64 CP 64^3-256^3, 1-32 timesteps",
+            "budget_gb": budget_gb, "requests_per_point": n_req,
+            "plain": {"simplices_per_s": plain_tp, "requests_per_s": plain_rate, "requests": plain_sample},
+            "points": points, "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"],
+            "saturated_simplices_per_s": sat["throughput_simplices_per_s"],
+            "speedup_vs_plain": sat["throughput_simplices_per_s"] / plain_tp,
+            "alone_equals_mixed": bool(exact)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +224,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 1024-job batching leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batch-jobs", type=int, default=1024)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (PMC calibration)")
     args = ap.parse_args()
@@ -164,6 +324,17 @@ def main():
                      "hbm_frac_total": total_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "hbm_frac_input_only": in_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    del dev, dev_views
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        out["batching"] = batch_leg(args.seed, args.batch_jobs, reps=3)
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        out["serving"] = serve_leg(args.seed, args.serve_requests, [0.5, 1, 2, 4, 8], args.serve_budget_gb,
+                                   plain_sample=24)
+        out["p50_ms"] = out["serving"]["p50_ms"]
+        out["p99_ms"] = out["serving"]["p99_ms"]
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         log(f"[bench] cpu baseline on a {args.cpu_sample}^3 x {nt} sample")
         out["cpu_baseline"] = cpu_baseline(comps, args.cpu_sample)

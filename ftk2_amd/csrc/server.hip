@@ -60,8 +60,9 @@ int n_types_of(int d, int m) { return n_types_rt(d, m); }
 
 // Device bytes reserved for a stream: its two resident timesteps, the per-slab
 // work arrays of a batched pass (u64 mask + u32 offset per slab vertex) and an
-// output allowance of 2% of the simplices becoming nodes (key, lambda, parent,
-// track, one edge each), over every slab it may append.
+// output allowance of one node per 256 simplices (key, lambda, parent, track, one
+// edge each) over every slab it may append.  Actual use is reported by
+// ftk2h_server_stats; an allocation beyond the device's memory fails with E_OOM.
 uint64_t reservation(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int ncomp, uint64_t max_t) {
   uint64_t nsp = 1;
   for (int j = 0; j < sd; ++j) nsp *= dims[j];
@@ -71,7 +72,7 @@ uint64_t reservation(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p
   const uint64_t work = 2 * nsp * 12;
   const uint64_t per_node = 8 + (uint64_t)(m + 1) * esz + 4 + 4 + 8;
   const uint64_t slabs = max_t > 1 ? max_t - 1 : 0;
-  const uint64_t out = slabs * nsp * (uint64_t)n_types_of(d, m) / 50 * per_node;
+  const uint64_t out = slabs * nsp * (uint64_t)n_types_of(d, m) / 256 * per_node;
   return fields + work + out;
 }
 

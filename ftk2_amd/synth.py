@@ -70,34 +70,42 @@ def scalar_config(n=384, nt=4, ngauss=64, seed=0, dtype=np.float32):
 def gaussian_gradient_device(spatial, nt, ngauss, sigma=(4.0, 10.0), amp=(0.5, 1.0), vel=1.0, seed=0,
                              device="cuda"):
     """Analytic gradient of a sum of moving Gaussians evaluated on the GPU (torch,
-    FP64): the same distributions as gaussians() (amplitudes +-U[amp], sigma
-    U[sigma], centres uniform in the domain, velocity U[-vel, vel] cells/step), for
-    request pools of the serving benchmark.  Returns len(spatial) tensors shaped
-    (nt, [nz,] ny, nx), each timestep a contiguous x-fastest block."""
+    FP64), with the distributions of gaussians() (amplitudes +-U[amp], sigma
+    U[sigma], centres uniform in the domain, velocity U[-vel, vel] cells/step); used
+    for the large request pools of the batching and serving benchmarks.  The
+    separable form sum_k prod_a e_ka(x_a) turns every component into batched matrix
+    products over the Gaussians.  Returns len(spatial) tensors shaped
+    (nt, [nz,] ny, nx): every timestep is a contiguous x-fastest block."""
     import torch
     rng = np.random.default_rng(seed)
     sd = len(spatial)
-    axes = [torch.arange(n, dtype=torch.float64, device=device) for n in spatial]
-    shape = tuple(reversed(spatial))
-    out = [torch.zeros((nt,) + shape, dtype=torch.float64, device=device) for _ in range(sd)]
     A = rng.uniform(*amp, size=ngauss) * np.where(rng.random(ngauss) < 0.5, -1.0, 1.0)
     S = rng.uniform(*sigma, size=ngauss)
     C0 = np.stack([rng.uniform(0, n - 1, size=ngauss) for n in spatial], axis=1)
     V = rng.uniform(-vel, vel, size=(ngauss, sd))
-    for t in range(nt):
-        for g in range(ngauss):
-            c = C0[g] + V[g] * t
-            s2 = S[g] * S[g]
-            d = [axes[a] - c[a] for a in range(sd)]
-            e = [torch.exp(-(d[a] * d[a]) / (2 * s2)) for a in range(sd)]
-            # separable: g(x) = prod_a e_a(x_a); broadcast with x the last (fastest) axis
-            def bc(v, a):
-                view = [1] * sd
-                view[sd - 1 - a] = -1
-                return v.view(view)
-            gval = bc(e[0], 0)
-            for a in range(1, sd):
-                gval = gval * bc(e[a], a)
-            for a in range(sd):
-                out[a][t] += (-A[g] / s2) * bc(d[a], a) * gval
+    f64 = dict(dtype=torch.float64, device=device)
+    t = torch.arange(nt, **f64)
+    s2 = torch.tensor(S * S, **f64)                                   # (K,)
+    a = torch.tensor(A, **f64)
+    E, W = [], []
+    for ax, n in enumerate(spatial):
+        x = torch.arange(n, **f64)
+        c = torch.tensor(C0[:, ax], **f64)[None, :] + t[:, None] * torch.tensor(V[:, ax], **f64)[None, :]  # (T, K)
+        d = x[None, None, :] - c[:, :, None]                          # (T, K, n)
+        e = torch.exp(-(d * d) / (2 * s2[None, :, None]))
+        E.append(e)
+        W.append((-a / s2)[None, :, None] * d * e)                    # derivative factor along ax
+    out = []
+    for ax in range(sd):
+        # component ax: W on axis ax, E on the others; x is the last (fastest) axis
+        f = [W[k] if k == ax else E[k] for k in range(sd)]
+        if sd == 1:
+            comp = f[0].sum(dim=1)
+        elif sd == 2:
+            comp = torch.bmm(f[1].transpose(1, 2).contiguous(), f[0].contiguous())  # (T, ny, nx)
+        else:
+            zy = (f[2][:, :, :, None] * f[1][:, :, None, :]).reshape(nt, ngauss, -1)  # (T, K, nz*ny)
+            comp = torch.bmm(zy.transpose(1, 2).contiguous(), f[0].contiguous())      # (T, nz*ny, nx)
+            comp = comp.reshape(nt, spatial[2], spatial[1], spatial[0])
+        out.append(comp.contiguous())
     return out

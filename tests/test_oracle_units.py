@@ -144,3 +144,4 @@ def test_encode_simplex_id_known():
     # tet with v_min = 0 on a 4x4x4x4 grid: vertices 0, 1, 1+4, 1+4+16
     code = oracle.encode_simplex_id([4, 4, 4, 4], [0, 1, 5, 21])
     assert code == (0 << 24) | (0b0001 << 8) | (0b0011 << 4) | 0b0111
+
