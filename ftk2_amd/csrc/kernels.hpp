@@ -1030,6 +1030,37 @@ __device__ __forceinline__ void sort_cells(uint32_t* seg, uint32_t n) {
   }
 }
 
+// Position of staged cell c inside its vertex's segment [s, e) of the staging
+// array: the number of the segment's cells ordered before it (lexicographic, equal
+// cells in staging order).  One thread per cell, so a vertex with many cells costs
+// its segment length per cell instead of a quadratic sort in one lane.
+template <int NV>
+__device__ __forceinline__ uint32_t cell_rank(const uint32_t* stage, uint32_t s, uint32_t e, uint32_t c) {
+  uint32_t r = 0;
+  if constexpr (NV == 2) {
+    const uint64_t kc = ((uint64_t)stage[2 * c] << 32) | stage[2 * c + 1];
+    for (uint32_t j = s; j < e; ++j) {
+      const uint64_t kj = ((uint64_t)stage[2 * j] << 32) | stage[2 * j + 1];
+      r += (kj < kc || (kj == kc && j < c)) ? 1u : 0u;
+    }
+  } else {
+    uint32_t a[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) a[q] = stage[c * NV + q];
+    for (uint32_t j = s; j < e; ++j) {
+      bool lt = false, eq = true;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const uint32_t b = stage[j * NV + q];
+        lt = lt || (eq && b < a[q]);
+        eq = eq && b == a[q];
+      }
+      r += (lt || (eq && j < c)) ? 1u : 0u;
+    }
+  }
+  return r;
+}
+
 // Emit: the nodes of a workgroup occupy one contiguous output range in canonical
 // order (vertex, then type), so they are queued in LDS in that order and written by
 // consecutive threads (coalesced, load balanced).  Cells are staged in LDS, each
@@ -1129,14 +1160,27 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
     const uint32_t ex = block_excl_scan(nc, &ctot);
     const uint64_t cbase = P.ebsum[blk];
     if (ctot * NV <= P.stage_cap) {
-      if (nc) {
-        uint32_t* seg = stage + ex * NV;
-        cells_of<KIND, D, M>(P, x, 1, seg);
-        sort_cells<NV>(seg, nc);
-      }
+      __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
+      if (nc) cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
+      segx[threadIdx.x] = ex;
+      if (threadIdx.x == 0) segx[BLOCK] = ctot;
       __syncthreads();
+      // each cell goes straight to its sorted place in the workgroup's output range
       uint32_t* dst = P.cells + cbase * NV;
-      for (uint32_t i = threadIdx.x; i < ctot * NV; i += BLOCK) dst[i] = stage[i];
+      for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
+        uint32_t lo = 0, hi = BLOCK;  // segx[lo] <= c < segx[hi]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (segx[mid] <= c)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        const uint32_t sb = segx[lo];
+        const uint32_t pos = sb + cell_rank<NV>(stage, sb, segx[lo + 1], c);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) dst[pos * NV + q] = stage[c * NV + q];
+      }
       // Union the cells whose nodes all belong to this workgroup in LDS and start
       // the global union-find from those roots (roots stay component minima).
       if (ntot <= EMIT_LUF) {
