@@ -157,6 +157,8 @@ int run_locked(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** 
   if ((rc = S.tile_base.ensure((na + 1) * 8))) return rc;
   if ((rc = S.tile_job.ensure(ntiles * 4))) return rc;
   if ((rc = S.mask.ensure(totv * 8))) return rc;
+  if ((rc = S.pend.ensure(totv * 8))) return rc;
+  if ((rc = S.degm.ensure(totv * 8))) return rc;
   if ((rc = S.node_off.ensure(totv * 4))) return rc;
   if ((rc = S.bsum.ensure((ntiles + 1) * 4))) return rc;
   if ((rc = S.ebsum.ensure((ntiles + 1) * 4))) return rc;
@@ -203,6 +205,8 @@ int run_locked(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** 
     if (res[j]->has_scalar && (rc = bind(*J.scalar, &J.mesh, nverts[j], &P.scal, P.sst))) return rc;
     finish_params(P, &J.mesh, S.ncomp);
     P.mask = static_cast<uint64_t*>(S.mask.p) + vert_base[a];
+    P.pend = static_cast<uint64_t*>(S.pend.p) + vert_base[a];
+    P.degm = static_cast<uint64_t*>(S.degm.p) + vert_base[a];
     P.node_off = static_cast<uint32_t*>(S.node_off.p) + vert_base[a];
     P.bsum = static_cast<uint32_t*>(S.bsum.p) + tile_base[a];
     P.ebsum = static_cast<uint32_t*>(S.ebsum.p) + tile_base[a];
@@ -342,7 +346,7 @@ void ftk2h_batch_destroy(ftk2h_batch* b) {
   if (S.st) (void)hipStreamSynchronize(S.st);
   if (S.g1) (void)hipGraphExecDestroy(S.g1);
   if (S.g2) (void)hipGraphExecDestroy(S.g2);
-  for (DevBuf* p : {&S.jobs, &S.tile_base, &S.tile_job, &S.mask, &S.node_off, &S.bsum, &S.ebsum, &S.sums,
+  for (DevBuf* p : {&S.jobs, &S.tile_base, &S.tile_job, &S.mask, &S.pend, &S.degm, &S.node_off, &S.bsum, &S.ebsum, &S.sums,
                     &S.counters, &S.bases, &S.xedges, &S.parent, &S.staging})
     p->release();
   if (S.host_bases) (void)hipHostFree(S.host_bases);

@@ -19,7 +19,8 @@ struct BatchDev {
   const uint32_t* tile_job;   // job of every tile (ntiles)
   const uint64_t* tile_base;  // first tile of every job (njobs + 1)
   uint64_t ntiles;
-  uint32_t* counters;         // dynamic schedule: one counter per phase
+  uint32_t* counters;         // dynamic schedule: one counter per kernel (0 classify, 1 offsets,
+                              // 2 emit, 4 stage 2); [3] counts cross-workgroup edges
   uint32_t chunk;             // tiles per grab; 0 = static grid stride
 };
 
@@ -53,6 +54,11 @@ __device__ __forceinline__ void for_tiles(const BatchDev& B, int phase, F&& f) {
 template <int KIND, int D, int M, class T>
 __global__ __launch_bounds__(BLOCK, 3) void kb_classify(BatchDev B) {
   for_tiles(B, 0, [](const KParams& P, int64_t blk) { classify_tile<KIND, D, M, T>(P, blk); });
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK, 4) void kb_stage2(BatchDev B) {
+  for_tiles(B, 4, [](const KParams& P, int64_t blk) { stage2_tile<KIND, D, M, T>(P, blk); });
 }
 
 template <int KIND, int D, int M, class T>
@@ -189,7 +195,7 @@ struct BatchState {
   hipStream_t st = nullptr;
   int n_cu = 256;
   BatchConfig cfg;
-  DevBuf jobs, tile_base, tile_job, mask, node_off, bsum, ebsum, sums, counters, bases, xedges, parent, staging;
+  DevBuf jobs, tile_base, tile_job, mask, pend, degm, node_off, bsum, ebsum, sums, counters, bases, xedges, parent, staging;
   uint32_t* host_bases = nullptr;  // pinned: node_base[njobs+1], cell_base[njobs+1]
   size_t host_bases_cap = 0;
   std::shared_ptr<Arena> arena;
@@ -242,8 +248,9 @@ int run_batch(BatchArgs& A, std::vector<ftk2h_result*>& outs) {
                             (uint64_t)(uintptr_t)S->host_bases, (uint64_t)(uintptr_t)S->tile_job.p,
                             (uint64_t)(uintptr_t)S->sums.p, ntiles, nj, grid_p, B.chunk};
   int rc = launch_graph_or_direct(S, &S->g1, &S->sig1, sig, [&]() -> int {
-    kb_zero<<<1, 64, 0, st>>>(B.counters, 4);
+    kb_zero<<<1, 64, 0, st>>>(B.counters, 5);
     kb_classify<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);
+    kb_stage2<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);
     HIP_TRY(hipGetLastError());
     HIP_TRY(scan_u32(bsum, bsum, ntiles, sums, st));
     kb_offsets<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);

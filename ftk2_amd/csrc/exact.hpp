@@ -138,6 +138,8 @@ __device__ __forceinline__ int tp_sign_d(const double* A, const double* B, const
   f = fma(e2, A[2], f);
   if (f > bnd) return 1;
   if (f < -bnd) return -1;
+  // a zero row or B parallel to C (the cross product is exact) gives exactly zero
+  if ((e0 == 0.0 && e1 == 0.0 && e2 == 0.0) || (A[0] == 0.0 && A[1] == 0.0 && A[2] == 0.0)) return 0;
   const int64_t s = (int64_t)e0 * (int64_t)A[0] + (int64_t)e1 * (int64_t)A[1] + (int64_t)e2 * (int64_t)A[2];
   return s > 0 ? 1 : (s < 0 ? -1 : 0);
 }

@@ -128,6 +128,8 @@ void ftk2h_ctx_destroy(ftk2h_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->mask.release();
+  ctx->pend.release();
+  ctx->degm.release();
   ctx->node_off.release();
   ctx->bsum.release();
   ctx->ebsum.release();
@@ -212,11 +214,15 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   const int64_t nb = (int64_t)((nvert + BLOCK - 1) / BLOCK);
   int rc;
   if ((rc = ctx->mask.ensure(nvert * 8))) return rc;
+  if ((rc = ctx->pend.ensure(nvert * 8))) return rc;
+  if ((rc = ctx->degm.ensure(nvert * 8))) return rc;
   if ((rc = ctx->node_off.ensure(nvert * 4))) return rc;
   if ((rc = ctx->bsum.ensure((nb + 1) * 4))) return rc;
   if ((rc = ctx->ebsum.ensure((nb + 1) * 4))) return rc;
   if ((rc = ctx->scan_sums.ensure((nb / SCAN_TILE + 2) * 4))) return rc;
   P.mask = static_cast<uint64_t*>(ctx->mask.p);
+  P.pend = static_cast<uint64_t*>(ctx->pend.p);
+  P.degm = static_cast<uint64_t*>(ctx->degm.p);
   P.node_off = static_cast<uint32_t*>(ctx->node_off.p);
   P.bsum = static_cast<uint32_t*>(ctx->bsum.p);
   P.ebsum = static_cast<uint32_t*>(ctx->ebsum.p);

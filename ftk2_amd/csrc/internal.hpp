@@ -70,7 +70,7 @@ constexpr int NPHASE = 7;
 struct ftk2h_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  ftk2h_internal::DevBuf mask, node_off, bsum, ebsum, xedges, xcount, scan_sums, fields[4];
+  ftk2h_internal::DevBuf mask, pend, degm, node_off, bsum, ebsum, xedges, xcount, scan_sums, fields[4];
   hipEvent_t ev[ftk2h_internal::NPHASE + 1] = {};
   std::mutex lock;
 };
@@ -309,7 +309,10 @@ int run_pipeline(RunArgs& A) {
   const int64_t nb = A.nb;
   auto rec = [&](int i) { return hipEventRecord(ctx->ev[i], st); };
   HIP_TRY(rec(0));
-  if (nb > 0) k_classify<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
+  if (nb > 0) {
+    k_classify<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
+    k_stage2<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(1));
   HIP_TRY(scan_u32(P.bsum, P.bsum, (uint64_t)nb, static_cast<uint32_t*>(ctx->scan_sums.p), st));

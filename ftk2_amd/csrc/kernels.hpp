@@ -55,6 +55,8 @@ struct KParams {
   uint32_t nq_cap;      // emit: queued nodes per workgroup (<= EMIT_NQ)
   uint32_t stage_cap;   // emit: staged cell words per workgroup (<= EMIT_STAGE_WORDS)
   uint64_t* mask;
+  uint64_t* pend;   // types left to stage 2 of the classification
+  uint64_t* degm;   // those among them that need the verbatim sign levels
   uint32_t* node_off;
   uint32_t* bsum;   // per-block node counts -> exclusive offsets (+ total at [nb])
   uint32_t* ebsum;  // per-block cell counts  -> exclusive offsets (+ total at [nb])
@@ -476,61 +478,67 @@ __device__ __forceinline__ Masks classify_fast(const KParams& P, const Vtx& x, c
     return classify_edge<KIND, T, D>(P, x);
 }
 
-// Stage 2: one pending simplex (type t owned by the vertex at l).  deg: re-run the
-// verbatim sign tests; then the solver gate for critical points.
+// Stage 2 helpers for one pending simplex (type t owned by the vertex of r).
+// Field values of its vertices (fiber: minus the thresholds).
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ bool eval_item(const KParams& P, const VtxRef& r, int t, bool deg) {
+__device__ __forceinline__ void item_values(const KParams& P, const VtxRef& r, int t, T (*vals)[M]) {
   const uint32_t pc = packed_chain<D, M>(t);
-  int off[M + 1];
-  off[0] = 0;
 #pragma unroll
-  for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
-  if constexpr (M == 1) {
-    T v0 = load_val<T>(P, 0, r, off[0]);
-    T v1 = load_val<T>(P, 0, r, off[1]);
-    T lam[2];
-    return solve_1<T>(v0, v1, lam);  // only CriticalPointPredicate<1> queues edges
+  for (int i = 0; i <= M; ++i) {
+    const int off = i == 0 ? 0 : (int)((pc >> (4 * (i - 1))) & 0xF);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      T v = load_val<T>(P, k, r, off);
+      if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
+      vals[i][k] = v;
+    }
+  }
+}
+
+// The verbatim SoS test (origin_inside<M>, sos.hpp:202-254) of a degenerate simplex.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ bool item_inside(const KParams& P, const VtxRef& r, int t) {
+  static_assert(M >= 2, "edges never take the verbatim path");
+  T vals[M + 1][M];
+  item_values<KIND, D, M, T>(P, r, t, vals);
+  double Xd[M + 1][M];
+  double mx = 0;
+#pragma unroll
+  for (int i = 0; i <= M; ++i)
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      Xd[i][k] = qt(vals[i][k], P.Q);
+      mx = fmax(mx, fabs(Xd[i][k]));
+    }
+  if constexpr (M == 3) {
+    if (mx < 0x1p20) return inside3_exact_d(Xd, mx);
+    int64_t X[M + 1][M];
+    for (int i = 0; i <= M; ++i)
+      for (int k = 0; k < M; ++k) X[i][k] = q64(vals[i][k], P.Q);
+    return inside3_exact(X);
   } else {
-    T vals[M + 1][M];
+    int64_t X[M + 1][M];
 #pragma unroll
     for (int i = 0; i <= M; ++i)
 #pragma unroll
-      for (int k = 0; k < M; ++k) {
-        T v = load_val<T>(P, k, r, off[i]);
-        if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
-        vals[i][k] = v;
-      }
-    if (deg) {
-      double Xd[M + 1][M];
-      double mx = 0;
-#pragma unroll
-      for (int i = 0; i <= M; ++i)
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-          Xd[i][k] = qt(vals[i][k], P.Q);
-          mx = fmax(mx, fabs(Xd[i][k]));
-        }
-      bool inside;
-      if constexpr (M == 3) {
-        if (mx < 0x1p20) {
-          inside = inside3_exact_d(Xd, mx);
-        } else {
-          int64_t X[M + 1][M];
-          for (int i = 0; i <= M; ++i)
-            for (int k = 0; k < M; ++k) X[i][k] = q64(vals[i][k], P.Q);
-          inside = inside3_exact(X);
-        }
-      } else {
-        int64_t X[M + 1][M];
-#pragma unroll
-        for (int i = 0; i <= M; ++i)
-#pragma unroll
-          for (int k = 0; k < M; ++k) X[i][k] = q64(vals[i][k], P.Q);
-        inside = inside2_exact(X);
-      }
-      if (!inside) return false;
-    }
-    if constexpr (KIND == KIND_FIBER) return true;
+      for (int k = 0; k < M; ++k) X[i][k] = q64(vals[i][k], P.Q);
+    return inside2_exact(X);
+  }
+}
+
+// The solver gate of CriticalPointPredicate (predicate.hpp:74): a failed solve
+// rejects the simplex.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ bool item_solves(const KParams& P, const VtxRef& r, int t) {
+  if constexpr (M == 1) {
+    const uint32_t pc = packed_chain<D, M>(t);
+    T v0 = load_val<T>(P, 0, r, 0);
+    T v1 = load_val<T>(P, 0, r, (int)(pc & 0xF));
+    T lam[2];
+    return solve_1<T>(v0, v1, lam);
+  } else {
+    T vals[M + 1][M];
+    item_values<KIND, D, M, T>(P, r, t, vals);
     T lam[M + 1];
     return solve_k<M, T>(vals, lam);
   }
@@ -886,57 +894,103 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
 // Tile functions: one workgroup processes the 256 vertices [blk*256, blk*256 + 256)
 // of the mesh described by P.  The single-mesh kernels run one tile per workgroup;
 // the batched kernels (batch.hip) loop over tiles of many meshes.
+// Stage 1: every vertex's fast classification.  mask = accepted types, pend =
+// types that need stage 2 (SoS inside with the solver gate still to run, or the
+// verbatim sign levels), degm = the verbatim ones among them.
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
-  constexpr int NTY = Chains<D, M>::N;
-  constexpr int QBYTES = BLOCK * NTY * 2;
-  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 0;
-  // stage 1 reads the quantized tile; stage 2 reuses the same bytes for its queue
-  __shared__ __align__(16) unsigned char smem[QBYTES > TBYTES ? QBYTES : TBYTES];
-  uint16_t* queue = reinterpret_cast<uint16_t*>(smem);
-  float* tile = reinterpret_cast<float*>(smem);
-  __shared__ unsigned long long slot[BLOCK];
-  __shared__ VtxRef refs[BLOCK];
-  __shared__ uint32_t qn;
-  if (threadIdx.x == 0) qn = 0;
+  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 16;
+  __shared__ __align__(16) float tile[TBYTES / 4];
   const int64_t v0 = blk * BLOCK;
   const int64_t v = v0 + threadIdx.x;
-  if constexpr (M == 3) stage_quantized3<T, D>(P, v0, tile);
-  __syncthreads();
-  uint64_t acc = 0, sol = 0, deg = 0;
+  if constexpr (M == 3) {
+    stage_quantized3<T, D>(P, v0, tile);
+    __syncthreads();
+  }
   if (v < P.nvert) {
     const Vtx x = make_vtx<D>(P, v);
-    refs[threadIdx.x] = x.r;
     const Masks mk = classify_fast<KIND, D, M, T>(P, x, tile);
-    acc = mk.acc;
-    sol = mk.sol;
-    deg = mk.deg;
+    P.mask[v] = mk.acc;
+    P.pend[v] = mk.sol | mk.deg;
+    P.degm[v] = mk.deg;
   }
-  if constexpr (M == 3) __syncthreads();  // the queue overwrites the tile
-  slot[threadIdx.x] = acc;
-  uint64_t w = sol | deg;
-  if (w) {
-    uint32_t pos = atomicAdd(&qn, (uint32_t)__popcll(w));
-    while (w) {
+}
+
+// Stage 2: the pending simplices of a tile, run from a workgroup queue in LDS (in
+// rounds of at most S2_QCAP items) so that every lane does useful work; then the
+// final mask and the tile's node count.
+constexpr int S2_QCAP = 2048;
+
+// Runs f(owner thread, type) over the items of every thread's bit set w (LDS
+// queue in rounds of at most S2_QCAP items, one item per lane).
+template <class F>
+__device__ __forceinline__ void run_items(uint16_t* queue, uint64_t w, F&& f) {
+  while (true) {
+    uint32_t tot;
+    uint32_t pos = block_excl_scan((uint32_t)__popcll(w), &tot);
+    if (tot == 0) break;
+    while (w && pos < (uint32_t)S2_QCAP) {
       const int t = __builtin_ctzll(w);
       w &= w - 1;
-      queue[pos++] = (uint16_t)(threadIdx.x | (t << 8) | (((deg >> t) & 1) << 14));
+      queue[pos++] = (uint16_t)(threadIdx.x | (t << 8));
     }
+    __syncthreads();
+    const uint32_t n = tot < (uint32_t)S2_QCAP ? tot : (uint32_t)S2_QCAP;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+      const uint32_t it = queue[i];
+      f((int)(it & 0xFF), (int)(it >> 8));
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const uint32_t n = qn;
-  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-    const uint32_t it = queue[i];
-    const int owner = it & 0xFF, t = (it >> 8) & 0x3F;
-    const bool dg = (it >> 14) & 1;
-    if (eval_item<KIND, D, M, T>(P, refs[owner], t, dg)) atomicOr(&slot[owner], 1ull << t);
+}
+
+// Phase A runs the verbatim sign tests of the degenerate items; for critical
+// points the ones found inside join the solver items of phase B.  Each phase
+// keeps a wave on one code path.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
+  __shared__ uint16_t queue[S2_QCAP];
+  __shared__ unsigned long long slot[BLOCK];
+  __shared__ unsigned long long prom[BLOCK];
+  __shared__ VtxRef refs[BLOCK];
+  const int64_t v = blk * BLOCK + threadIdx.x;
+  uint64_t acc = 0, w = 0, deg = 0;
+  if (v < P.nvert) {
+    acc = P.mask[v];
+    w = P.pend[v];
+    deg = P.degm[v];
   }
-  __syncthreads();
-  const uint64_t m = slot[threadIdx.x];
-  if (v < P.nvert) P.mask[v] = m;
+  if (__syncthreads_or(w != 0)) {
+    if (w) refs[threadIdx.x] = make_vtx<D>(P, v).r;
+    slot[threadIdx.x] = acc;
+    prom[threadIdx.x] = 0;
+    if constexpr (M >= 2) {
+      run_items(queue, w & deg, [&](int owner, int t) {
+        if (item_inside<KIND, D, M, T>(P, refs[owner], t)) {
+          if constexpr (KIND == KIND_FIBER)
+            atomicOr(&slot[owner], 1ull << t);
+          else
+            atomicOr(&prom[owner], 1ull << t);
+        }
+      });
+    }
+    if constexpr (KIND != KIND_FIBER) {
+      const uint64_t ws = (w & ~deg) | prom[threadIdx.x];
+      run_items(queue, ws, [&](int owner, int t) {
+        if (item_solves<KIND, D, M, T>(P, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
+      });
+    }
+    acc = slot[threadIdx.x];
+    if (v < P.nvert) P.mask[v] = acc;
+  }
   uint32_t tot;
-  block_excl_scan((uint32_t)__popcll(m), &tot);
+  block_excl_scan((uint32_t)__popcll(acc), &tot);
   if (threadIdx.x == 0) P.bsum[blk] = tot;
+}
+
+template <int KIND, int D, int M, class T>
+__global__ __launch_bounds__(BLOCK, 4) void k_stage2(KParams P) {
+  stage2_tile<KIND, D, M, T>(P, blockIdx.x);
 }
 
 template <int KIND, int D, int M, class T>
