@@ -199,6 +199,30 @@ __device__ __forceinline__ uint32_t packed_chain(int t) {
   return P.v[t];
 }
 
+// Per-corner element offsets and the chain table, copied to LDS once per tile for
+// the kernels that index them with per-lane (run-time) corner and type numbers;
+// from kernel arguments such indexing would cost a vector load per access.
+struct CornerTab {
+  int64_t coff[3][16];
+  int64_t soff[16];
+  uint32_t pc[64];
+};
+template <int D, int M>
+__device__ __forceinline__ void load_corner_tab(const KParams& P, CornerTab* tab) {
+  for (int i = threadIdx.x; i < 48 + 16 + 64; i += BLOCK) {
+    if (i < 48)
+      tab->coff[i / 16][i % 16] = P.coff[i / 16][i % 16];
+    else if (i < 64)
+      tab->soff[i - 48] = P.soff[i - 48];
+    else
+      tab->pc[i - 64] = packed_chain<D, M>(i - 64);
+  }
+}
+template <class T>
+__device__ __forceinline__ T load_val_t(const KParams& P, const CornerTab& tab, int c, const VtxRef& r, int s) {
+  return static_cast<const T*>(P.comp[c])[r.base[c] + tab.coff[c][s]];
+}
+
 // Table references usable inside lambdas without capture.
 template <int D, int M>
 constexpr const ChainTable<D, M>& CH = Chains<D, M>::table;
@@ -481,14 +505,15 @@ __device__ __forceinline__ Masks classify_fast(const KParams& P, const Vtx& x, c
 // Stage 2 helpers for one pending simplex (type t owned by the vertex of r).
 // Field values of its vertices (fiber: minus the thresholds).
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ void item_values(const KParams& P, const VtxRef& r, int t, T (*vals)[M]) {
-  const uint32_t pc = packed_chain<D, M>(t);
+__device__ __forceinline__ void item_values(const KParams& P, const CornerTab& tab, const VtxRef& r, int t,
+                                            T (*vals)[M]) {
+  const uint32_t pc = tab.pc[t];
 #pragma unroll
   for (int i = 0; i <= M; ++i) {
     const int off = i == 0 ? 0 : (int)((pc >> (4 * (i - 1))) & 0xF);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-      T v = load_val<T>(P, k, r, off);
+      T v = load_val_t<T>(P, tab, k, r, off);
       if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
       vals[i][k] = v;
     }
@@ -497,10 +522,10 @@ __device__ __forceinline__ void item_values(const KParams& P, const VtxRef& r, i
 
 // The verbatim SoS test (origin_inside<M>, sos.hpp:202-254) of a degenerate simplex.
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ bool item_inside(const KParams& P, const VtxRef& r, int t) {
+__device__ __forceinline__ bool item_inside(const KParams& P, const CornerTab& tab, const VtxRef& r, int t) {
   static_assert(M >= 2, "edges never take the verbatim path");
   T vals[M + 1][M];
-  item_values<KIND, D, M, T>(P, r, t, vals);
+  item_values<KIND, D, M, T>(P, tab, r, t, vals);
   double Xd[M + 1][M];
   double mx = 0;
 #pragma unroll
@@ -529,16 +554,16 @@ __device__ __forceinline__ bool item_inside(const KParams& P, const VtxRef& r, i
 // The solver gate of CriticalPointPredicate (predicate.hpp:74): a failed solve
 // rejects the simplex.
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ bool item_solves(const KParams& P, const VtxRef& r, int t) {
+__device__ __forceinline__ bool item_solves(const KParams& P, const CornerTab& tab, const VtxRef& r, int t) {
   if constexpr (M == 1) {
-    const uint32_t pc = packed_chain<D, M>(t);
-    T v0 = load_val<T>(P, 0, r, 0);
-    T v1 = load_val<T>(P, 0, r, (int)(pc & 0xF));
+    const uint32_t pc = tab.pc[t];
+    T v0 = load_val_t<T>(P, tab, 0, r, 0);
+    T v1 = load_val_t<T>(P, tab, 0, r, (int)(pc & 0xF));
     T lam[2];
     return solve_1<T>(v0, v1, lam);
   } else {
     T vals[M + 1][M];
-    item_values<KIND, D, M, T>(P, r, t, vals);
+    item_values<KIND, D, M, T>(P, tab, r, t, vals);
     T lam[M + 1];
     return solve_k<M, T>(vals, lam);
   }
@@ -547,17 +572,17 @@ __device__ __forceinline__ bool item_solves(const KParams& P, const VtxRef& r, i
 // ---------------------------------------------------------------------------
 // lambda of one node (type t at vertex l), written as T[M+1]; returns scalar.
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ void node_lambda(const KParams& P, const VtxRef& r, int t, T* lam,
-                                            float* scalar) {
-  const uint32_t pc = packed_chain<D, M>(t);
+__device__ __forceinline__ void node_lambda(const KParams& P, const CornerTab& tab, const VtxRef& r, int t,
+                                            T* lam, float* scalar) {
+  const uint32_t pc = tab.pc[t];
   int off[M + 1];
   off[0] = 0;
 #pragma unroll
   for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
   *scalar = 0.f;
   if constexpr (M == 1) {
-    T v0 = load_val<T>(P, 0, r, off[0]);
-    T v1 = load_val<T>(P, 0, r, off[1]);
+    T v0 = load_val_t<T>(P, tab, 0, r, off[0]);
+    T v1 = load_val_t<T>(P, tab, 0, r, off[1]);
     if constexpr (KIND == KIND_CONTOUR) {
       v0 = v0 - (T)P.thr[0];
       v1 = v1 - (T)P.thr[0];
@@ -572,7 +597,7 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const VtxRef& r, i
     for (int i = 0; i <= M; ++i)
 #pragma unroll
       for (int k = 0; k < M; ++k) {
-        vals[i][k] = load_val<T>(P, k, r, off[i]);
+        vals[i][k] = load_val_t<T>(P, tab, k, r, off[i]);
         if constexpr (KIND == KIND_FIBER) vals[i][k] = vals[i][k] - (T)P.thr[k];
       }
     const bool ok = solve_k<M, T>(vals, lam);
@@ -586,7 +611,7 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const VtxRef& r, i
     if (P.scal) {  // predicate.hpp:81-92
       T sv = T(0);
 #pragma unroll
-      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * load_scalar<T>(P, r, off[i]);
+      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * static_cast<const T*>(P.scal)[r.sbase + tab.soff[off[i]]];
       *scalar = (float)sv;
     }
   }
@@ -953,6 +978,8 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
   __shared__ unsigned long long slot[BLOCK];
   __shared__ unsigned long long prom[BLOCK];
   __shared__ VtxRef refs[BLOCK];
+  __shared__ CornerTab tab;
+  load_corner_tab<D, M>(P, &tab);
   const int64_t v = blk * BLOCK + threadIdx.x;
   uint64_t acc = 0, w = 0, deg = 0;
   if (v < P.nvert) {
@@ -966,7 +993,7 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
     prom[threadIdx.x] = 0;
     if constexpr (M >= 2) {
       run_items(queue, w & deg, [&](int owner, int t) {
-        if (item_inside<KIND, D, M, T>(P, refs[owner], t)) {
+        if (item_inside<KIND, D, M, T>(P, tab, refs[owner], t)) {
           if constexpr (KIND == KIND_FIBER)
             atomicOr(&slot[owner], 1ull << t);
           else
@@ -977,7 +1004,7 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
     if constexpr (KIND != KIND_FIBER) {
       const uint64_t ws = (w & ~deg) | prom[threadIdx.x];
       run_items(queue, ws, [&](int owner, int t) {
-        if (item_solves<KIND, D, M, T>(P, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
+        if (item_solves<KIND, D, M, T>(P, tab, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
       });
     }
     acc = slot[threadIdx.x];
@@ -1158,6 +1185,9 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   __shared__ uint16_t nq[EMIT_NQ];
   __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t stage[EMIT_STAGE_WORDS];
+  __shared__ CornerTab tab;
+  load_corner_tab<D, M>(P, &tab);
+  __syncthreads();
   const int64_t v = blk * BLOCK + threadIdx.x;
   Vtx x;
   uint64_t m = 0;
@@ -1175,7 +1205,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   auto emit_node = [&](uint32_t pos, const VtxRef& r, int t) {
     T lam[M + 1];
     float sc;
-    node_lambda<KIND, D, M, T>(P, r, t, lam, &sc);
+    node_lambda<KIND, D, M, T>(P, tab, r, t, lam, &sc);
     P.keys[pos] = r.gid * (uint64_t)NTY + (uint64_t)t;
 #pragma unroll
     for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
