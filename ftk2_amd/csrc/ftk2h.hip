@@ -118,6 +118,11 @@ int ftk2h_ctx_create(int device, ftk2h_ctx** out) {
   ctx->device = device;
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  ctx->pool = std::make_shared<DevPool>();
+  ctx->pool->device = device;
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  ctx->pool->cap = total_b / 8;  // idle result blocks kept for reuse
   for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
   *out = ctx.release();
   return FTK2H_OK;
@@ -136,6 +141,8 @@ void ftk2h_ctx_destroy(ftk2h_ctx* ctx) {
   ctx->xedges.release();
   ctx->xcount.release();
   ctx->scan_sums.release();
+  ctx->parent.release();
+  if (ctx->pool) ctx->pool->trim(0);  // results still alive keep the pool
   for (auto& f : ctx->fields) f.release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);

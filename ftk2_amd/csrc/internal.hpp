@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <map>
 
 namespace ftk2h_internal {
 using namespace ftk2h;
@@ -65,6 +66,76 @@ struct DevBuf {
 
 constexpr int NPHASE = 7;
 
+// Cache of device blocks for result arrays: a freed result returns its blocks
+// here and the next extraction of a similar size takes them back instead of
+// calling hipMalloc/hipFree (hundreds of microseconds each for GB-sized arrays).
+// Bounded by `cap` bytes of idle blocks; shared by the context and its results.
+struct DevPool {
+  int device = 0;
+  size_t cap = 0, idle = 0;
+  std::mutex m;
+  std::multimap<size_t, void*> free_blocks;  // capacity -> block
+  std::map<void*, size_t> owned;              // every block this pool handed out
+  hipError_t get(void** out, size_t n) {
+    {
+      std::lock_guard<std::mutex> g(m);
+      auto it = free_blocks.lower_bound(n);
+      if (it != free_blocks.end() && it->first <= 2 * n + (1u << 20)) {
+        *out = it->second;
+        idle -= it->first;
+        free_blocks.erase(it);
+        return hipSuccess;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e == hipErrorOutOfMemory) {  // give the idle blocks back and retry once
+      trim(0);
+      e = hipMalloc(&p, n ? n : 1);
+    }
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(m);
+    owned[p] = n ? n : 1;
+    *out = p;
+    return hipSuccess;
+  }
+  void put(void* p) {
+    if (!p) return;
+    std::unique_lock<std::mutex> g(m);
+    auto it = owned.find(p);
+    if (it == owned.end() || idle + it->second > cap) {
+      if (it != owned.end()) owned.erase(it);
+      g.unlock();
+      free_on_device(p);
+      return;
+    }
+    free_blocks.emplace(it->second, p);
+    idle += it->second;
+  }
+  void trim(size_t keep) {
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> g(m);
+      while (idle > keep && !free_blocks.empty()) {
+        auto it = std::prev(free_blocks.end());
+        idle -= it->first;
+        owned.erase(it->second);
+        drop.push_back(it->second);
+        free_blocks.erase(it);
+      }
+    }
+    for (void* p : drop) free_on_device(p);
+  }
+  void free_on_device(void* p) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    (void)hipFree(p);
+    (void)hipSetDevice(cur);
+  }
+  ~DevPool() { trim(0); }
+};
+
 }  // namespace ftk2h_internal
 
 struct ftk2h_ctx {
@@ -72,6 +143,8 @@ struct ftk2h_ctx {
   hipStream_t stream = nullptr;
   ftk2h_internal::DevBuf mask, pend, degm, node_off, bsum, ebsum, xedges, xcount, scan_sums, fields[4];
   hipEvent_t ev[ftk2h_internal::NPHASE + 1] = {};
+  std::shared_ptr<ftk2h_internal::DevPool> pool;
+  ftk2h_internal::DevBuf parent;  // union-find scratch
   std::mutex lock;
 };
 
@@ -94,18 +167,28 @@ struct ftk2h_result {
   // Set when the arrays are views into memory shared with other results (batched
   // runs); the memory is released with its last holder.
   std::shared_ptr<void> arena;
-  ~ftk2h_result() {
-    if (arena) return;
+  // Device blocks of keys/lambda/track/scalar/conn come from this pool (or, when
+  // it is null, from hipMalloc directly).
+  std::shared_ptr<ftk2h_internal::DevPool> pool;
+  void release(void* p) {
+    if (!p) return;
+    if (pool) {
+      pool->put(p);
+      return;
+    }
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    if (keys) (void)hipFree(keys);
-    if (lambda) (void)hipFree(lambda);
-    if (track) (void)hipFree(track);
-    if (scalar) (void)hipFree(scalar);
-    for (auto* c : conn)
-      if (c) (void)hipFree(c);
+    (void)hipFree(p);
     (void)hipSetDevice(cur);
+  }
+  ~ftk2h_result() {
+    if (arena) return;
+    release(keys);
+    release(lambda);
+    release(track);
+    release(scalar);
+    for (auto* c : conn) release(c);
   }
 };
 
@@ -293,7 +376,7 @@ int sort_unique_cells(ftk2h_result* R, uint64_t nn, uint64_t nc, hipStream_t st)
   if ((rc = check(hipMalloc(&out, (uint64_t)std::max<uint32_t>(nu, 1) * NV * 4), "hipMalloc"))) return rc;
   k_cell_compact<NV><<<gc, BLOCK, 0, st>>>(tmp, keep, pos, nc, out);
   if ((rc = check(hipStreamSynchronize(st), "sync"))) return rc;
-  (void)hipFree(R->conn[0]);
+  R->release(R->conn[0]);
   R->conn[0] = out;
   R->conn_cells[0] = nu;
   cleanup();
@@ -339,12 +422,15 @@ int run_pipeline(RunArgs& A) {
   R->n_nodes = nn;
   R->ntypes = Chains<D, M>::N;
   uint32_t* parent = nullptr;
+  R->pool = ctx->pool;
+  DevPool& pool = *ctx->pool;
   if (nn) {
-    HIP_TRY(hipMalloc(&R->keys, nn * 8));
-    HIP_TRY(hipMalloc(&R->lambda, nn * (M + 1) * sizeof(T)));
-    HIP_TRY(hipMalloc(&R->track, nn * 4));
-    HIP_TRY(hipMalloc(&parent, nn * 4));
-    if (P.scal) HIP_TRY(hipMalloc(&R->scalar, nn * 4));
+    HIP_TRY(pool.get(reinterpret_cast<void**>(&R->keys), nn * 8));
+    HIP_TRY(pool.get(&R->lambda, nn * (M + 1) * sizeof(T)));
+    HIP_TRY(pool.get(reinterpret_cast<void**>(&R->track), nn * 4));
+    if (P.scal) HIP_TRY(pool.get(reinterpret_cast<void**>(&R->scalar), nn * 4));
+    if (int rc = ctx->parent.ensure(nn * 4)) return rc;
+    parent = static_cast<uint32_t*>(ctx->parent.p);
   }
   if (NV > 0 && nc) {
     int rc = ctx->xedges.ensure(nc * (uint64_t)(NV - 1) * 8);
@@ -353,7 +439,7 @@ int run_pipeline(RunArgs& A) {
     R->n_conn = 1;
     R->conn_dim[0] = NV - 1;
     R->conn_cells[0] = nc;
-    HIP_TRY(hipMalloc(&R->conn[0], nc * NV * 4));
+    HIP_TRY(pool.get(reinterpret_cast<void**>(&R->conn[0]), nc * NV * 4));
   }
   P.keys = R->keys;
   P.lambda = R->lambda;
@@ -388,9 +474,7 @@ int run_pipeline(RunArgs& A) {
       if (rc) return rc;
     }
   }
-  // parent is scratch; free after the stream drains
   HIP_TRY(hipStreamSynchronize(st));
-  if (parent) HIP_TRY(hipFree(parent));
   for (int i = 0; i < NPHASE; ++i) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]) == hipSuccess) R->phase_ms[i] = ms;

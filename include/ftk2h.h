@@ -88,7 +88,9 @@ typedef struct {
                              when the struct is NULL) */
 } ftk2h_options;
 
-/* Device-resident view of a result (pointers stay owned by the result). */
+/* Device-resident view of a result (pointers stay owned by the result and valid
+ * until ftk2h_result_free; device work that reads them must be complete by then,
+ * since the library may reuse the memory for its next extraction). */
 typedef struct {
   uint64_t n_nodes;
   int node_dim;              /* m */
