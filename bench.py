@@ -125,7 +125,7 @@ def batch_leg(seed, njobs, reps, log_every=256):
             "speedup": plain_s / t_b, "bit_exact_sample": bool(exact) and batch_nodes == n_nodes}
 
 
-def serve_leg(seed, n_req, rates_x, budget_gb, plain_sample):
+def serve_leg(seed, n_req, rates_x, budget_gb):
     """Serving: Poisson arrivals of streaming-tracking requests, 70% 2D FP64 CP at
     128^2-2048^2 and 30% 3D FP64 CP at 64^3-256^3, 1-32 timesteps each, admitted
     against a device-memory budget and continuously batched.  Offered rate swept in
@@ -143,8 +143,7 @@ def serve_leg(seed, n_req, rates_x, budget_gb, plain_sample):
     torch.cuda.synchronize()
     log(f"[serve] request pools generated in {time.perf_counter() - t0:.1f} s")
 
-    def make(nreq, rate, rs):
-        arr = serve.poisson_arrivals(nreq, rate, rs) if rate else np.zeros(nreq)
+    def make(nreq, rs):
         out = []
         for i in range(nreq):
             if rs.random() < 0.7:
@@ -156,61 +155,69 @@ def serve_leg(seed, n_req, rates_x, budget_gb, plain_sample):
             perm = tuple(reversed(range(sd)))
             steps = [[p[t].permute(*perm) for p in pool] for t in range(T)]
             simp = (n - 1) ** sd * max(0, T - 1) * (12 if sd == 2 else 60)
-            out.append(serve.Request(i, float(arr[i]), (n,) * sd, _lib.PRED_CRITICAL_POINT, sd, _lib.F64, steps,
+            out.append(serve.Request(i, 0.0, (n,) * sd, _lib.PRED_CRITICAL_POINT, sd, _lib.F64, steps,
                                      simplices=simp, meta={"sd": sd, "n": n, "T": T}))
         return out
 
-    # plain: the same kind of requests one at a time through ftk2h_stream
-    plain_reqs = make(plain_sample, None, np.random.default_rng(seed + 1))
-    t0 = time.perf_counter()
-    for rq in plain_reqs:
+    # one request set; the plain run and every offered rate serve exactly these
+    reqs = make(n_req, np.random.default_rng(seed + 1))
+    work = sum(r.simplices for r in reqs)
+
+    def alone(rq):
         st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]))
         for comps in rq.timesteps:
             st.append(comps)
         r = st.finish()
-        del r
         st.close()
+        return r
+
+    alone(reqs[0])  # warm
+    t0 = time.perf_counter()
+    for rq in reqs:
+        r = alone(rq)
+        del r
     plain_s = time.perf_counter() - t0
-    plain_work = sum(r.simplices for r in plain_reqs)
-    plain_rate = plain_sample / plain_s  # requests/s one at a time
-    log(f"[serve] plain: {plain_sample} requests in {plain_s:.2f} s ({plain_rate:.1f} req/s)")
+    plain_rate = n_req / plain_s
+    log(f"[serve] plain: {n_req} requests one at a time in {plain_s:.2f} s ({plain_rate:.1f} req/s, "
+        f"{work / plain_s:.3e} simplices/s)")
     points = []
     for k, x in enumerate(rates_x):
         rate = x * plain_rate
-        reqs = make(n_req, rate, np.random.default_rng(seed * 1000 + k))
+        arr = serve.poisson_arrivals(n_req, rate, np.random.default_rng(seed * 1000 + k))
+        for rq, a in zip(reqs, arr):
+            rq.arrival = float(a)
         srv = engine.Server(int(budget_gb * (1 << 30)))
         outs, wall, rej = serve.run(srv, reqs)
         pt = serve.summarize(outs, reqs, wall)
         pt.update({"offered_req_per_s": rate, "offered_x_plain": x, "rejected": len(rej)})
         points.append(pt)
         log(f"[serve] offered {rate:.1f} req/s ({x}x plain): p50 {pt['p50_ms']:.1f} ms p99 {pt['p99_ms']:.1f} ms "
-            f"{pt['throughput_simplices_per_s']:.3e} simplices/s")
+            f"{pt['requests_per_s']:.1f} req/s {pt['throughput_simplices_per_s']:.3e} simplices/s")
         srv.close()
-    # each result the same alone and in the mix: a few requests served together vs alone
-    check = make(6, None, np.random.default_rng(seed + 99))
-    check = [r for r in check if r.meta["n"] <= 512] or check[:2]
+    # each result the same alone and in the mix (requests up to 512^2 / 128^3)
+    check = [r for r in reqs if (r.meta["sd"] == 2 and r.meta["n"] <= 512) or
+             (r.meta["sd"] == 3 and r.meta["n"] <= 128)][:8]
+    for rq in check:
+        rq.arrival = 0.0
     srv = engine.Server(int(budget_gb * (1 << 30)))
     outs, _, _ = serve.run(srv, check, keep_results=True)
     exact = len(outs) == len(check)
     by_rid = {r.rid: r for r in check}
     for o in outs:
-        rq = by_rid[o.rid]
-        st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]))
-        for comps in rq.timesteps:
-            st.append(comps)
-        a, b = o.result.to_complex(), st.finish().to_complex()
-        st.close()
+        a, b = o.result.to_complex(), alone(by_rid[o.rid]).to_complex()
         exact = exact and np.array_equal(a.vertices, b.vertices) and np.array_equal(a.lam, b.lam) and \
-            np.array_equal(a.track_id, b.track_id)
+            np.array_equal(a.track_id, b.track_id) and all(
+                np.array_equal(a.connectivity[d], b.connectivity[d]) for d in b.connectivity)
     srv.close()
     sat = max(points, key=lambda p: p["throughput_simplices_per_s"])
-    plain_tp = plain_work / plain_s
+    plain_tp = work / plain_s
     return {"workload": "Poisson streams: 70% 2D FP64 CP 128^2-2048^2, 30% 3D FP64 CP 64^3-256^3, 1-32 timesteps",
             "budget_gb": budget_gb, "requests_per_point": n_req,
-            "plain": {"simplices_per_s": plain_tp, "requests_per_s": plain_rate, "requests": plain_sample},
+            "plain": {"simplices_per_s": plain_tp, "requests_per_s": plain_rate, "wall_s": plain_s},
             "points": points, "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"],
             "saturated_simplices_per_s": sat["throughput_simplices_per_s"],
             "speedup_vs_plain": sat["throughput_simplices_per_s"] / plain_tp,
+            "requests_per_s_vs_plain": sat["requests_per_s"] / plain_rate,
             "alone_equals_mixed": bool(exact)}
 
 
@@ -227,7 +234,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 1024-job batching leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--batch-jobs", type=int, default=1024)
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (PMC calibration)")
@@ -330,8 +337,7 @@ def main():
         out["batching"] = batch_leg(args.seed, args.batch_jobs, reps=3)
         torch.cuda.empty_cache()
     if not args.no_serve:
-        out["serving"] = serve_leg(args.seed, args.serve_requests, [0.5, 1, 2, 4, 8], args.serve_budget_gb,
-                                   plain_sample=24)
+        out["serving"] = serve_leg(args.seed, args.serve_requests, [0.5, 1, 2, 4, 8], args.serve_budget_gb)
         out["p50_ms"] = out["serving"]["p50_ms"]
         out["p99_ms"] = out["serving"]["p99_ms"]
         torch.cuda.empty_cache()

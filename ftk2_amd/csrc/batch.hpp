@@ -24,51 +24,50 @@ struct BatchDev {
   uint32_t chunk;             // tiles per grab; 0 = static grid stride
 };
 
-// Runs f(P, local tile) for every tile given to this workgroup.
+// Runs f(P, local tile) for every tile given to this workgroup.  The job table
+// comes in as its own restrict-qualified kernel argument: nothing the kernel
+// stores can alias it, so the compiler reads a job's parameters with scalar loads
+// (as it reads kernel arguments) instead of per-lane vector loads.
 template <class F>
-__device__ __forceinline__ void for_tiles(const BatchDev& B, int phase, F&& f) {
-  if (B.chunk == 0) {
-    for (uint64_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
-      __syncthreads();  // LDS of the previous tile is no longer read
-      const uint32_t j = B.tile_job[t];
-      f(B.jobs[j], (int64_t)(t - B.tile_base[j]));
-    }
-    return;
-  }
+__device__ __forceinline__ void for_tiles(const BatchDev& B, const KParams* __restrict__ jobs, int phase, F&& f) {
+  // one call site of f, so that the tile body is inlined into the kernel
   __shared__ uint32_t grab;
+  const bool dyn = B.chunk != 0;
+  uint64_t t = dyn ? 0 : blockIdx.x, end = 0;
   while (true) {
-    __syncthreads();
-    if (threadIdx.x == 0) grab = atomicAdd(B.counters + phase, B.chunk);
-    __syncthreads();
-    const uint64_t t0 = __builtin_amdgcn_readfirstlane(grab);
-    if (t0 >= B.ntiles) break;
-    const uint64_t t1 = t0 + B.chunk < B.ntiles ? t0 + B.chunk : B.ntiles;
-    for (uint64_t t = t0; t < t1; ++t) {
+    if (dyn && t >= end) {  // take the next chunk of consecutive tiles
       __syncthreads();
-      const uint32_t j = B.tile_job[t];
-      f(B.jobs[j], (int64_t)(t - B.tile_base[j]));
+      if (threadIdx.x == 0) grab = atomicAdd(B.counters + phase, B.chunk);
+      __syncthreads();
+      t = __builtin_amdgcn_readfirstlane(grab);
+      end = t + B.chunk;
     }
+    if (t >= B.ntiles) break;
+    __syncthreads();  // LDS of the previous tile is no longer read
+    const uint32_t j = B.tile_job[t];
+    f(jobs[j], (int64_t)(t - B.tile_base[j]));
+    t += dyn ? 1 : gridDim.x;
   }
 }
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK, 3) void kb_classify(BatchDev B) {
-  for_tiles(B, 0, [](const KParams& P, int64_t blk) { classify_tile<KIND, D, M, T>(P, blk); });
+__global__ __launch_bounds__(BLOCK, 3) void kb_classify(BatchDev B, const KParams* __restrict__ jobs) {
+  for_tiles(B, jobs, 0, [](const KParams& P, int64_t blk) __attribute__((always_inline)) { classify_tile<KIND, D, M, T>(P, blk); });
 }
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK, 4) void kb_stage2(BatchDev B) {
-  for_tiles(B, 4, [](const KParams& P, int64_t blk) { stage2_tile<KIND, D, M, T>(P, blk); });
+__global__ __launch_bounds__(BLOCK, 4) void kb_stage2(BatchDev B, const KParams* __restrict__ jobs) {
+  for_tiles(B, jobs, 4, [](const KParams& P, int64_t blk) __attribute__((always_inline)) { stage2_tile<KIND, D, M, T>(P, blk); });
 }
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK) void kb_offsets(BatchDev B) {
-  for_tiles(B, 1, [](const KParams& P, int64_t blk) { offsets_tile<KIND, D, M, T>(P, blk); });
+__global__ __launch_bounds__(BLOCK) void kb_offsets(BatchDev B, const KParams* __restrict__ jobs) {
+  for_tiles(B, jobs, 1, [](const KParams& P, int64_t blk) __attribute__((always_inline)) { offsets_tile<KIND, D, M, T>(P, blk); });
 }
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK) void kb_emit(BatchDev B) {
-  for_tiles(B, 2, [](const KParams& P, int64_t blk) { emit_tile<KIND, D, M, T>(P, blk); });
+__global__ __launch_bounds__(BLOCK) void kb_emit(BatchDev B, const KParams* __restrict__ jobs) {
+  for_tiles(B, jobs, 2, [](const KParams& P, int64_t blk) __attribute__((always_inline)) { emit_tile<KIND, D, M, T>(P, blk); });
 }
 
 // tile -> job map
@@ -249,11 +248,11 @@ int run_batch(BatchArgs& A, std::vector<ftk2h_result*>& outs) {
                             (uint64_t)(uintptr_t)S->sums.p, ntiles, nj, grid_p, B.chunk};
   int rc = launch_graph_or_direct(S, &S->g1, &S->sig1, sig, [&]() -> int {
     kb_zero<<<1, 64, 0, st>>>(B.counters, 5);
-    kb_classify<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);
-    kb_stage2<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);
+    kb_classify<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B, B.jobs);
+    kb_stage2<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B, B.jobs);
     HIP_TRY(hipGetLastError());
     HIP_TRY(scan_u32(bsum, bsum, ntiles, sums, st));
-    kb_offsets<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);
+    kb_offsets<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B, B.jobs);
     HIP_TRY(hipGetLastError());
     HIP_TRY(scan_u32(ebsum, ebsum, ntiles, sums, st));
     kb_job_bases<<<dim3((nj + BLOCK) / BLOCK), dim3(BLOCK), 0, st>>>(B.tile_base, nj, bsum, ebsum, node_base,
@@ -304,7 +303,7 @@ int run_batch(BatchArgs& A, std::vector<ftk2h_result*>& outs) {
         static_cast<KParams*>(S->jobs.p), nj, keys, lambda, scal, cells, xedges, parent, track, nn, nc);
     HIP_TRY(hipGetLastError());
     if (nn) {
-      kb_emit<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B);
+      kb_emit<KIND, D, M, T><<<dim3((unsigned)grid_p), dim3(BLOCK), 0, st>>>(B, B.jobs);
       HIP_TRY(hipGetLastError());
       if constexpr (NV > 1) {
         if (nc) {

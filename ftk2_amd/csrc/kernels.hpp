@@ -72,6 +72,15 @@ struct KParams {
   uint64_t n_cells;
 };
 
+// Global-memory view of a device pointer.  Pointers the kernels read out of memory
+// (the batched job table) are generic to the compiler, which would then use flat
+// instructions; the arrays of KParams always live in global memory.
+#define FTK2H_AS1 __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ FTK2H_AS1 T* gptr(T* p) {
+  return (FTK2H_AS1 T*)p;
+}
+
 // ---------------------------------------------------------------------------
 // compile-time loop
 template <int I, int N, class F>
@@ -169,12 +178,12 @@ __device__ __forceinline__ uint32_t corner_mask(const KParams& P, const int64_t*
 
 template <class T>
 __device__ __forceinline__ T load_val(const KParams& P, int c, const VtxRef& r, int s) {
-  return static_cast<const T*>(P.comp[c])[r.base[c] + P.coff[c][s]];
+  return gptr(static_cast<const T*>(P.comp[c]))[r.base[c] + P.coff[c][s]];
 }
 
 template <class T>
 __device__ __forceinline__ T load_scalar(const KParams& P, const VtxRef& r, int s) {
-  return static_cast<const T*>(P.scal)[r.sbase + P.soff[s]];
+  return gptr(static_cast<const T*>(P.scal))[r.sbase + P.soff[s]];
 }
 
 // chains packed 4 bits per vertex offset, for run-time type lookups
@@ -220,7 +229,7 @@ __device__ __forceinline__ void load_corner_tab(const KParams& P, CornerTab* tab
 }
 template <class T>
 __device__ __forceinline__ T load_val_t(const KParams& P, const CornerTab& tab, int c, const VtxRef& r, int s) {
-  return static_cast<const T*>(P.comp[c])[r.base[c] + tab.coff[c][s]];
+  return gptr(static_cast<const T*>(P.comp[c]))[r.base[c] + tab.coff[c][s]];
 }
 
 // Table references usable inside lambdas without capture.
@@ -611,7 +620,7 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const CornerTab& t
     if (P.scal) {  // predicate.hpp:81-92
       T sv = T(0);
 #pragma unroll
-      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * static_cast<const T*>(P.scal)[r.sbase + tab.soff[off[i]]];
+      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * gptr(static_cast<const T*>(P.scal))[r.sbase + tab.soff[off[i]]];
       *scalar = (float)sv;
     }
   }
@@ -679,8 +688,8 @@ __device__ __forceinline__ void sort_small(uint32_t* c) {
       }
 }
 
-template <int KIND, int D, int M>
-__device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int mode, uint32_t* dst) {
+template <int KIND, int D, int M, class Ptr = uint32_t*>
+__device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int mode, Ptr dst) {
   constexpr int CK = CellKind<KIND, D, M>::value;
   if constexpr (CK == 0) {
     return 0;
@@ -697,8 +706,8 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
       no[o] = 0;
       if ((OWN >> o) & 1) {
         const int64_t u = x.v + P.loff[o];
-        mk[o] = P.mask[u];
-        if (mode) no[o] = P.node_off[u];
+        mk[o] = gptr(P.mask)[u];
+        if (mode) no[o] = gptr(P.node_off)[u];
       }
     }
     auto idx = [&](int o, int t) -> uint32_t {
@@ -935,9 +944,9 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
   if (v < P.nvert) {
     const Vtx x = make_vtx<D>(P, v);
     const Masks mk = classify_fast<KIND, D, M, T>(P, x, tile);
-    P.mask[v] = mk.acc;
-    P.pend[v] = mk.sol | mk.deg;
-    P.degm[v] = mk.deg;
+    gptr(P.mask)[v] = mk.acc;
+    gptr(P.pend)[v] = mk.sol | mk.deg;
+    gptr(P.degm)[v] = mk.deg;
   }
 }
 
@@ -983,9 +992,9 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
   const int64_t v = blk * BLOCK + threadIdx.x;
   uint64_t acc = 0, w = 0, deg = 0;
   if (v < P.nvert) {
-    acc = P.mask[v];
-    w = P.pend[v];
-    deg = P.degm[v];
+    acc = gptr(P.mask)[v];
+    w = gptr(P.pend)[v];
+    deg = gptr(P.degm)[v];
   }
   if (__syncthreads_or(w != 0)) {
     if (w) refs[threadIdx.x] = make_vtx<D>(P, v).r;
@@ -1008,11 +1017,11 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
       });
     }
     acc = slot[threadIdx.x];
-    if (v < P.nvert) P.mask[v] = acc;
+    if (v < P.nvert) gptr(P.mask)[v] = acc;
   }
   uint32_t tot;
   block_excl_scan((uint32_t)__popcll(acc), &tot);
-  if (threadIdx.x == 0) P.bsum[blk] = tot;
+  if (threadIdx.x == 0) gptr(P.bsum)[blk] = tot;
 }
 
 template <int KIND, int D, int M, class T>
@@ -1064,18 +1073,18 @@ __device__ __forceinline__ void offsets_tile(const KParams& P, int64_t blk) {
   uint64_t m = 0;
   uint32_t nc = 0;
   if (v < P.nvert) {
-    m = P.mask[v];
+    m = gptr(P.mask)[v];
     Vtx x;
     x.v = v;
     decompose<D>(P, v, x.l);
-    if (is_hc_base<D>(P, x.l)) nc = cells_of<KIND, D, M>(P, x, 0, nullptr);
+    if (is_hc_base<D>(P, x.l)) nc = cells_of<KIND, D, M>(P, x, 0, (uint32_t*)nullptr);
   }
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
-  if (v < P.nvert) P.node_off[v] = P.bsum[blk] + ex;
+  if (v < P.nvert) gptr(P.node_off)[v] = gptr(P.bsum)[blk] + ex;
   if (blk == 0 && threadIdx.x == 0) *P.xcount = 0;
   block_excl_scan(nc, &tot);
-  if (threadIdx.x == 0) P.ebsum[blk] = tot;
+  if (threadIdx.x == 0) gptr(P.ebsum)[blk] = tot;
 }
 
 template <int KIND, int D, int M, class T>
@@ -1084,8 +1093,8 @@ __global__ __launch_bounds__(BLOCK) void k_offsets(KParams P) {
 }
 
 // Lexicographic insertion sort of n cells of NV indices (small n).
-template <int NV>
-__device__ __forceinline__ void sort_cells(uint32_t* seg, uint32_t n) {
+template <int NV, class Ptr>
+__device__ __forceinline__ void sort_cells(Ptr seg, uint32_t n) {
   for (uint32_t i = 1; i < n; ++i) {
     uint32_t cur[NV];
 #pragma unroll
@@ -1194,26 +1203,26 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   bool hc = false;
   if (v < P.nvert) {
     x = make_vtx<D>(P, v);
-    m = P.mask[v];
+    m = gptr(P.mask)[v];
     hc = is_hc_base<D>(P, x.l);
     refs[threadIdx.x] = x.r;
   }
   // ---- nodes
-  const uint32_t nbase = P.bsum[blk];
-  const uint32_t ntot = P.bsum[blk + 1] - nbase;
-  T* lam_out = static_cast<T*>(P.lambda);
+  const uint32_t nbase = gptr(P.bsum)[blk];
+  const uint32_t ntot = gptr(P.bsum)[blk + 1] - nbase;
+  auto lam_out = gptr(static_cast<T*>(P.lambda));
   auto emit_node = [&](uint32_t pos, const VtxRef& r, int t) {
     T lam[M + 1];
     float sc;
     node_lambda<KIND, D, M, T>(P, tab, r, t, lam, &sc);
-    P.keys[pos] = r.gid * (uint64_t)NTY + (uint64_t)t;
+    gptr(P.keys)[pos] = r.gid * (uint64_t)NTY + (uint64_t)t;
 #pragma unroll
     for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
-    if (P.scalar_out) P.scalar_out[pos] = sc;
+    if (P.scalar_out) gptr(P.scalar_out)[pos] = sc;
   };
   if (ntot <= P.nq_cap) {
     if (m) {
-      uint32_t pos = (v < P.nvert ? P.node_off[v] : 0) - nbase;
+      uint32_t pos = (v < P.nvert ? gptr(P.node_off)[v] : 0) - nbase;
       uint64_t mm = m;
       while (mm) {
         const int t = __builtin_ctzll(mm);
@@ -1227,7 +1236,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       emit_node(nbase + i, refs[it & 0xFF], (int)(it >> 8));
     }
   } else if (m) {
-    uint32_t pos = P.node_off[v];
+    uint32_t pos = gptr(P.node_off)[v];
     uint64_t mm = m;
     while (mm) {
       const int t = __builtin_ctzll(mm);
@@ -1239,10 +1248,10 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   // ---- cells
   if constexpr (NV > 0) {
     uint32_t nc = 0;
-    if (hc) nc = cells_of<KIND, D, M>(P, x, 0, nullptr);
+    if (hc) nc = cells_of<KIND, D, M>(P, x, 0, (uint32_t*)nullptr);
     uint32_t ctot;
     const uint32_t ex = block_excl_scan(nc, &ctot);
-    const uint64_t cbase = P.ebsum[blk];
+    const uint64_t cbase = gptr(P.ebsum)[blk];
     if (ctot * NV <= P.stage_cap) {
       __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
       if (nc) cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
@@ -1250,7 +1259,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       if (threadIdx.x == 0) segx[BLOCK] = ctot;
       __syncthreads();
       // each cell goes straight to its sorted place in the workgroup's output range
-      uint32_t* dst = P.cells + cbase * NV;
+      auto dst = gptr(P.cells) + cbase * NV;
       for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
         uint32_t lo = 0, hi = BLOCK;  // segx[lo] <= c < segx[hi]
         while (hi - lo > 1) {
@@ -1281,7 +1290,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
           }
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) P.parent[nbase + i] = nbase + lds_find(lpar, i);
+        for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) gptr(P.parent)[nbase + i] = nbase + lds_find(lpar, i);
         // edges leaving the workgroup, from the local root of their first node
         uint32_t nx = 0;
         for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
@@ -1302,8 +1311,8 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
           for (int q = 1; q < NV; ++q) {
             const uint32_t b = stage[c * NV + q];
             if (a >= ntot || b - nbase >= ntot) {
-              P.xedges[2 * (uint64_t)xo] = ra;
-              P.xedges[2 * (uint64_t)xo + 1] = b;
+              gptr(P.xedges)[2 * (uint64_t)xo] = ra;
+              gptr(P.xedges)[2 * (uint64_t)xo + 1] = b;
               xo++;
             }
           }
@@ -1311,7 +1320,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
         linked = true;
       }
     } else if (nc) {
-      uint32_t* seg = P.cells + (cbase + ex) * NV;
+      auto seg = gptr(P.cells) + (cbase + ex) * NV;
       cells_of<KIND, D, M>(P, x, 1, seg);
       sort_cells<NV>(seg, nc);
     }
@@ -1322,16 +1331,16 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       __shared__ uint32_t xbase2;
       if (threadIdx.x == 0) xbase2 = ne ? atomicAdd(P.xcount, ne) : 0;
       __syncthreads();
-      const uint32_t* src = P.cells + cbase * NV;
+      auto src = gptr(P.cells) + cbase * NV;
       for (uint32_t e = threadIdx.x; e < ne; e += BLOCK) {
         const uint32_t c = e / (NV - 1), q = 1 + e % (NV - 1);
-        P.xedges[2 * (uint64_t)(xbase2 + e)] = src[c * NV];
-        P.xedges[2 * (uint64_t)(xbase2 + e) + 1] = src[c * NV + q];
+        gptr(P.xedges)[2 * (uint64_t)(xbase2 + e)] = src[c * NV];
+        gptr(P.xedges)[2 * (uint64_t)(xbase2 + e) + 1] = src[c * NV + q];
       }
     }
   }
   if (!linked)
-    for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) P.parent[nbase + i] = nbase + i;
+    for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) gptr(P.parent)[nbase + i] = nbase + i;
 }
 
 template <int KIND, int D, int M, class T>

@@ -51,6 +51,9 @@ struct ftk2h_server {
   uint64_t next_id = 1;
   std::map<uint64_t, ServedStream> streams;
   std::map<BatchKey, ftk2h_batch*> batches;
+  DevBuf counts;                    // per-slab lower-plane node counts of a step
+  std::vector<uint32_t> host_counts;
+  hipStream_t st = nullptr;
   std::mutex lock;
 };
 
@@ -98,6 +101,8 @@ int ftk2h_server_create(ftk2h_ctx* ctx, uint64_t hbm_budget_bytes, ftk2h_server*
   auto s = std::make_unique<ftk2h_server>();
   s->ctx = ctx;
   s->budget = hbm_budget_bytes;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
   *out = s.release();
   return FTK2H_OK;
 }
@@ -228,8 +233,16 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
     std::vector<ftk2h_result*> res(list.size(), nullptr);
     int rc = ftk2h_batch_run(b, (int)jobs.size(), jobs.data(), res.data(), nullptr);
     if (rc) return rc;
+    // lower-plane node counts of every slab: one kernel each, one copy back
+    if ((rc = s->counts.ensure(list.size() * 4))) return rc;
+    if (s->host_counts.size() < list.size()) s->host_counts.resize(list.size());
+    uint32_t* dc = static_cast<uint32_t*>(s->counts.p);
+    for (size_t i = 0; i < list.size(); ++i) stream_launch_lower_count(list[i].S->s, res[i], dc + i, s->st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->host_counts.data(), dc, list.size() * 4, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
     for (size_t i = 0; i < list.size(); ++i) {
-      rc = stream_take_slab(list[i].S->s, res[i]);
+      rc = stream_take_slab_n(list[i].S->s, res[i], s->host_counts[i]);
       res[i] = nullptr;
       if (rc) {
         for (auto* r : res)
@@ -289,6 +302,8 @@ void ftk2h_server_destroy(ftk2h_server* s) {
     if (S.result) ftk2h_result_free(S.result);
   }
   for (auto& [k, b] : s->batches) ftk2h_batch_destroy(b);
+  s->counts.release();
+  if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }
 

@@ -70,6 +70,7 @@ __global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, ui
   const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nc) return;
   uint32_t c[NV];
+  bool ok = true;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     const uint32_t x = cells[i * NV + q];
@@ -85,8 +86,9 @@ __global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, ui
         else
           hi = mid;
       }
-      if (lo >= next_nlow || next_keys[lo] != key) {
+      if (lo >= next_nlow || next_keys[lo] != key) {  // cannot happen for a consistent pair of slabs
         atomicAdd(bad, 1u);
+        ok = false;
         lo = 0;
       }
       c[q] = next_base + lo;
@@ -94,6 +96,7 @@ __global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, ui
   }
 #pragma unroll
   for (int q = 0; q < NV; ++q) out[i * NV + q] = c[q];
+  if (!ok) return;  // reported by ftk2h_stream_finish; indices may lie beyond the parents
 #pragma unroll
   for (int q = 1; q < NV; ++q) uf_unite(parent, c[0], c[q]);
 }
@@ -169,7 +172,6 @@ int commit_open(ftk2h_stream* S, const ftk2h_result* next, uint32_t next_nlow) {
     const uint64_t nc = R->conn_cells[0];
     if ((rc = S->cells.reserve((S->committed_cells + nc) * S->nv * 4, S->st))) return rc;
     uint32_t* dst = static_cast<uint32_t*>(S->cells.p) + S->committed_cells * S->nv;
-    HIP_TRY(hipMemsetAsync(S->scratch + 1, 0, 4, S->st));
     const unsigned g = (unsigned)((nc + BLOCK - 1) / BLOCK);
     const uint64_t* nkeys = next ? next->keys : nullptr;
     const uint32_t nb = S->committed + nk;
@@ -179,10 +181,6 @@ int commit_open(ftk2h_stream* S, const ftk2h_result* next, uint32_t next_nlow) {
       case 4: k_remap_cells<4><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
     }
     HIP_TRY(hipGetLastError());
-    uint32_t bad = 0;
-    HIP_TRY(hipMemcpyAsync(&bad, S->scratch + 1, 4, hipMemcpyDeviceToHost, S->st));
-    HIP_TRY(hipStreamSynchronize(S->st));
-    if (bad) return fail(FTK2H_E_HIP, "stream: upper-plane node missing from the next slab");
     S->committed_cells += nc;
     S->cells.bytes = S->committed_cells * S->nv * 4;
   }
@@ -268,13 +266,26 @@ void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_vi
 // R = extraction of the pending slab (ownership passes to the stream): commit the
 // previously open slab against it and keep R open.
 int stream_take_slab(ftk2h_stream* S, ftk2h_result* R) {
-  const uint64_t slab_t = S->t - 2;
   uint32_t nlow = 0;
-  int rc = slab_lower_count(S, R, slab_t, &nlow);
+  int rc = slab_lower_count(S, R, S->t - 2, &nlow);
   if (rc) {
     ftk2h_result_free(R);
     return rc;
   }
+  return stream_take_slab_n(S, R, nlow);
+}
+
+uint64_t stream_lower_bound_key(const ftk2h_stream* S) { return (S->t - 1) * S->nsp * (uint64_t)S->ntypes; }
+
+void stream_launch_lower_count(const ftk2h_stream* S, const ftk2h_result* R, uint32_t* out, hipStream_t st) {
+  k_lower_bound<<<1, 1, 0, st>>>(R->keys, R->n_nodes, stream_lower_bound_key(S), out);
+}
+
+// nlow = nodes of R on the slab's lower time plane.  The re-indexing kernels are
+// queued on the stream's HIP stream; a missing upper-plane node is reported by
+// ftk2h_stream_finish.
+int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow) {
+  int rc;
   if (S->open) {
     rc = commit_open(S, R, nlow);
     ftk2h_result_free(S->open);
@@ -332,6 +343,7 @@ int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial
   HIP_TRY(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
   for (int c = 0; c < ncomp; ++c) HIP_TRY(hipMalloc(&S->fields[c], 2 * S->nsp * S->esz));
   HIP_TRY(hipMalloc(&S->scratch, 64));
+  HIP_TRY(hipMemset(S->scratch, 0, 64));  // [1] counts cells whose upper node was not found
   *out = S.release();
   return FTK2H_OK;
 }
@@ -376,6 +388,10 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
     S->open = nullptr;
     if (rc) return rc;
   }
+  uint32_t bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, S->scratch + 1, 4, hipMemcpyDeviceToHost, S->st));
+  HIP_TRY(hipStreamSynchronize(S->st));
+  if (bad) return fail(FTK2H_E_HIP, "stream: upper-plane node missing from the next slab");
   auto res = std::make_unique<ftk2h_result>();
   res->device = S->ctx->device;
   res->kind = S->pred.kind;

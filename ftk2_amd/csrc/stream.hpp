@@ -13,6 +13,10 @@ bool stream_slab_ready(const ftk2h_stream* S);
 void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_view* views);
 // Hand the pending slab's extraction to the stream (takes ownership of R).
 int stream_take_slab(ftk2h_stream* S, ftk2h_result* R);
+// The same with the count of R's nodes on the slab's lower time plane already
+// known (stream_launch_lower_count, for many slabs with one copy back).
+int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow);
+void stream_launch_lower_count(const ftk2h_stream* S, const ftk2h_result* R, uint32_t* out, hipStream_t st);
 uint64_t stream_timesteps(const ftk2h_stream* S);
 // Device memory held by the stream (timestep halves + committed nodes/cells).
 uint64_t stream_device_bytes(const ftk2h_stream* S);
