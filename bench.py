@@ -26,8 +26,9 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "cp3d_256x4_fp64_hbm_traffic.md
 
 
 def classify_traffic(n, nt):
-    """HBM bytes per k_classify launch measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE
-    passes recorded in the committed summary (256^3 x 4 only; None otherwise)."""
+    """HBM bytes of one classification (k_classify + k_stage2) measured by the rocprofv3
+    FETCH_SIZE/WRITE_SIZE passes recorded in the committed summary (256^3 x 4 only;
+    None otherwise)."""
     if (n, nt) != (256, 4) or not os.path.exists(TRAFFIC_SUMMARY):
         return None
     import re
@@ -298,8 +299,9 @@ def main():
     kernel_ms = float(ph.sum())
     classify_ms = float(ph[0])
     emit_ms = float(ph[4])
-    dominant = "classify" if classify_ms >= emit_ms else "emit"
-    if dominant == "classify":
+    # classify = stage 1 (k_classify) + stage 2 (k_stage2), the predicate evaluation
+    dominant = "k_classify+k_stage2" if classify_ms >= emit_ms else "k_emit"
+    if dominant == "k_classify+k_stage2":
         dom_bytes, dom_ms = in_bytes, classify_ms
     else:
         dom_bytes, dom_ms = n_nodes * (8 + 4 * 8) + n_edges * 8, emit_ms
@@ -322,7 +324,7 @@ def main():
                    "dup_frac": 0.01, "seed": args.seed, "nodes": int(n_nodes), "edges": int(n_edges)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": classify_traffic(n, nt) if dominant == "classify" else None,
+                     "traffic": classify_traffic(n, nt) if dominant == "k_classify+k_stage2" else None,
                      "algorithmic_bytes": int(dom_bytes)},
         "pipeline": {"kernel_ms": kernel_ms, "phase_ms": [float(x) for x in ph],
                      "phases": ["classify", "scan_nodes", "offsets+cell_count", "scan_cells+sync",

@@ -24,6 +24,9 @@
  *                              engine.hpp:999-1009 (hguo/ndarray, un-vendored)
  *   ftk2h_stream_*             SimplicialEngine::execute_stream engine.hpp:214-270 and
  *                              execute_cuda_streaming_regular :601-819
+ *   ftk2h_batch_*, ftk2h_server_*  no counterpart: batching of many meshes and a
+ *                              request server over streams (each result equals the
+ *                              single-mesh / single-stream entry point's)
  */
 #ifndef FTK2H_H
 #define FTK2H_H
