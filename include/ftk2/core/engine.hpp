@@ -135,6 +135,73 @@ class SimplicialEngine {
     }
   }
 
+  // engine.hpp:214-270 — timesteps read one at a time; each slab [t, t+1] is
+  // extracted on the device as it arrives (ftk2h_stream_*), the last timestep and
+  // the open union-find frontier stay resident.  The result equals execute() over
+  // the whole spacetime mesh.
+  void execute_stream(ftk::stream<>& s, const std::vector<std::string>& var_names = {}) {
+    (void)var_names;  // the predicate's own variable names select the arrays
+    const int n = s.total_timesteps();
+    if (n < 2) return;  // as the reference: nothing extracted, earlier results kept
+    clear_results();
+    if constexpr (!detail::pred_traits<PredicateType>::supported) {
+      throw std::runtime_error("SimplicialEngine: predicate not supported on the GPU");
+    } else {
+      std::vector<std::string> names;
+      ftk2h_predicate_desc pd{};
+      pd.kind = detail::pred_traits<PredicateType>::kind;
+      pd.codimension = m;
+      pd.dtype = std::is_same_v<T, double> ? FTK2H_F64 : FTK2H_F32;
+      pd.sos_q = predicate_.sos_q;
+      if constexpr (detail::pred_traits<PredicateType>::kind == FTK2H_PRED_CRITICAL_POINT) {
+        for (int j = 0; j < m; ++j) names.push_back(predicate_.var_names[j]);
+      } else if constexpr (detail::pred_traits<PredicateType>::kind == FTK2H_PRED_CONTOUR) {
+        names.push_back(predicate_.var_name);
+        pd.thresholds[0] = (double)predicate_.threshold;
+      } else {
+        for (int j = 0; j < 2; ++j) {
+          names.push_back(predicate_.var_names[j]);
+          pd.thresholds[j] = (double)predicate_.thresholds[j];
+        }
+      }
+      std::vector<uint64_t> spatial;
+      ftk2h_stream* st = nullptr;
+      struct Closer {
+        ftk2h_stream*& st;
+        ~Closer() {
+          if (st) ftk2h_stream_close(st);
+        }
+      } closer{st};
+      for (int t = 0; t < n; ++t) {
+        auto group = s.read(t);
+        std::vector<ftk2h_field_view> views;
+        for (const auto& name : names) {
+          const auto& a = group->template get_ref<T>(name);
+          if (t == 0 && spatial.empty())
+            for (size_t j = 0; j < a.nd(); ++j) spatial.push_back(a.dimf(j));
+          if (a.nd() != spatial.size()) throw std::runtime_error("execute_stream: timesteps differ in shape");
+          ftk2h_field_view v{};
+          v.ptr = a.data();
+          int64_t str = 1;
+          for (size_t j = 0; j < a.nd(); ++j) {
+            if (a.dimf(j) != spatial[j]) throw std::runtime_error("execute_stream: timesteps differ in shape");
+            v.strides[j] = str;
+            str *= (int64_t)a.dimf(j);
+          }
+          v.on_device = 0;
+          views.push_back(v);
+        }
+        if (t == 0) {
+          if (spatial.empty() || spatial.size() > 3) throw std::runtime_error("execute_stream: 1-3 spatial dims");
+          detail::check(ftk2h_stream_open(detail::context(device_), (int)spatial.size(), spatial.data(), &pd,
+                                          (int)names.size(), &st));
+        }
+        detail::check(ftk2h_stream_append(st, views.data(), nullptr));
+      }
+      detail::check(ftk2h_stream_finish(st, &result_));
+    }
+  }
+
   // engine.hpp:822 — vertices sorted by simplex, track_id = minimum sorted index of
   // the component, connectivity sorted and unique.
   FeatureComplex get_complex() {

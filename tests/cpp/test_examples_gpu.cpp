@@ -113,10 +113,56 @@ static void critical_point_3d() {  // test_examples.cpp:153-186 (see tests/test_
   std::printf("critical_point_3d: %zu vertices\n", cx.vertices.size());
 }
 
+// test_examples.cpp:259-297: streamed execution equals one-shot execution.  The
+// reference builds its field with hguo/ndarray's synthetic_woven_2Dt (not available
+// here); this uses a woven-like analytic field of the same shape and threshold.
+static void streaming() {
+  const int W = 16, H = 16, Tn = 5;
+  auto value = [](int x, int y, int t) {
+    const double s = (double)t / (Tn - 1);
+    return std::cos(0.4 * x + s) * std::sin(0.35 * y - s) + 0.25 * std::sin(0.7 * (x + y) * s);
+  };
+  ftk::ndarray<double> scalar({(size_t)W, (size_t)H, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) scalar.f(x, y, t) = value(x, y, t);
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{(uint64_t)W, (uint64_t)H, (uint64_t)Tn});
+  ContourPredicate<double> pred;
+  pred.var_name = "scalar";
+  pred.threshold = 0.5;
+  SimplicialEngine<double, ContourPredicate<double>> engine(mesh, pred);
+  engine.execute({{"scalar", scalar}});
+  auto one = engine.get_complex();
+
+  ftk::stream<> stream(Tn, [&](int t) {
+    auto g = std::make_shared<ftk::ndarray_group>();
+    ftk::ndarray<double> a({(size_t)W, (size_t)H});
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) a.f(x, y) = value(x, y, t);
+    g->set("scalar", a);
+    return g;
+  });
+  engine.execute_stream(stream);
+  auto streamed = engine.get_complex();
+  CHECK_EQ(streamed.vertices.size(), one.vertices.size());
+  CHECK_EQ(streamed.vertices.size() > 0, true);
+  for (size_t i = 0; i < one.vertices.size() && i < streamed.vertices.size(); ++i) {
+    CHECK_EQ(streamed.vertices[i].simplex < one.vertices[i].simplex, false);
+    CHECK_EQ(one.vertices[i].simplex < streamed.vertices[i].simplex, false);
+    CHECK_EQ(streamed.vertices[i].track_id, one.vertices[i].track_id);
+  }
+  CHECK_EQ(streamed.connectivity.size(), one.connectivity.size());
+  for (size_t c = 0; c < one.connectivity.size() && c < streamed.connectivity.size(); ++c)
+    CHECK_EQ(streamed.connectivity[c].indices == one.connectivity[c].indices, true);
+  check_complex(streamed);
+  std::printf("streaming: %zu vertices streamed, %zu one-shot\n", streamed.vertices.size(), one.vertices.size());
+}
+
 int main() {
   levelset_3d();
   fiber_3d();
   critical_point_3d();
+  streaming();
   std::printf("%d failures\n", failures);
   return failures;
 }
