@@ -49,6 +49,10 @@ class DeviceView(C.Structure):
                 ("conn", C.c_void_p * 3)]
 
 
+class AttributeDesc(C.Structure):
+    _fields_ = [("slot", C.c_int), ("ncomp", C.c_int), ("comps", FieldView * 16)]
+
+
 class Job(C.Structure):
     _fields_ = [("mesh", MeshDesc), ("comps", C.POINTER(FieldView)), ("scalar", C.POINTER(FieldView))]
 
@@ -87,6 +91,9 @@ SIGNATURES = {
                                        C.POINTER(C.c_float)]),
     "ftk2h_batch_last_ms": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
     "ftk2h_batch_destroy": (None, [_P]),
+    "ftk2h_result_add_attributes": (C.c_int, [_P, _P, C.POINTER(MeshDesc), C.POINTER(AttributeDesc), C.c_int,
+                                              _P]),
+    "ftk2h_result_copy_attributes": (C.c_int, [_P, _P]),
     "ftk2h_server_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
     "ftk2h_server_open": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(PredicateDesc), C.c_int,
                                     C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -158,6 +158,7 @@ struct ftk2h_result {
   void* lambda = nullptr;
   uint32_t* track = nullptr;
   float* scalar = nullptr;
+  float* attrs = nullptr;  // n_nodes * 16 (FeatureElement::attributes), when requested
   int n_conn = 0;
   int conn_dim[3] = {0, 0, 0};
   uint64_t conn_cells[3] = {0, 0, 0};
@@ -183,6 +184,7 @@ struct ftk2h_result {
     (void)hipSetDevice(cur);
   }
   ~ftk2h_result() {
+    release(attrs);  // never part of a batch's shared memory
     if (arena) return;
     release(keys);
     release(lambda);

@@ -168,6 +168,26 @@ class Result:
             _lib.check(_lib.lib().ftk2h_result_copy_keys(self.h, k.ctypes.data))
         return k
 
+    def add_attributes(self, mesh, specs, device: int = 0):
+        """specs: [(slot, [arrays indexed by local coordinates])] (several arrays: norm)."""
+        arr = (_lib.AttributeDesc * max(1, len(specs)))()
+        for i, (slot, arrays) in enumerate(specs):
+            arr[i].slot = slot
+            arr[i].ncomp = len(arrays)
+            for c, a in enumerate(arrays):
+                arr[i].comps[c] = field_view(a)
+        md = mesh.desc()
+        _lib.check(_lib.lib().ftk2h_result_add_attributes(context(device), self.h, C.byref(md), arr, len(specs),
+                                                          None))
+        self._attr_keep = specs
+
+    def attributes(self):
+        n, _, _ = self.sizes()
+        out = np.zeros((n, 16), np.float32)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_attributes(self.h, out.ctypes.data))
+        return out
+
     def to_complex(self):
         n, m, nconn = self.sizes()
         nv = m + 1

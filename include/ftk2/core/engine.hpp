@@ -86,8 +86,6 @@ class SimplicialEngine {
     if constexpr (!detail::pred_traits<PredicateType>::supported) {
       throw std::runtime_error("SimplicialEngine: predicate not supported on the GPU");
     } else {
-      if (!predicate_.attributes.empty())
-        throw std::runtime_error("SimplicialEngine: attribute recording is not supported on the GPU path");
       const auto& ld = reg->get_local_dims();
       const auto& off = reg->get_offset();
       const auto& gd = reg->get_global_dims();
@@ -132,6 +130,7 @@ class SimplicialEngine {
       }
       detail::check(ftk2h_extract_regular(detail::context(device_), &md, &pd, views.data(), (int)views.size(),
                                           has_scalar ? &scalar : nullptr, nullptr, &result_, nullptr));
+      add_attributes(data, reg.get(), md);
     }
   }
 
@@ -143,6 +142,10 @@ class SimplicialEngine {
     (void)var_names;  // the predicate's own variable names select the arrays
     const int n = s.total_timesteps();
     if (n < 2) return;  // as the reference: nothing extracted, earlier results kept
+    if constexpr (detail::pred_traits<PredicateType>::supported) {
+      if (!predicate_.attributes.empty())
+        throw std::runtime_error("SimplicialEngine: attribute recording is not supported by execute_stream");
+    }
     clear_results();
     if constexpr (!detail::pred_traits<PredicateType>::supported) {
       throw std::runtime_error("SimplicialEngine: predicate not supported on the GPU");
@@ -214,9 +217,12 @@ class SimplicialEngine {
     std::vector<uint64_t> verts(n * nv), track(n);
     std::vector<float> bary(n * nv), scal(n);
     if (n) detail::check(ftk2h_result_copy_nodes(result_, verts.data(), nullptr, bary.data(), scal.data(), track.data()));
+    std::vector<float> attrs(n * 16);
+    if (n) detail::check(ftk2h_result_copy_attributes(result_, attrs.data()));
     cx.vertices.resize(n);
     for (uint64_t i = 0; i < n; ++i) {
       FeatureElement& el = cx.vertices[i];
+      for (int k = 0; k < 16; ++k) el.attributes[k] = attrs[i * 16 + k];
       el.simplex.dimension = dim;
       for (int k = 0; k < nv; ++k) {
         el.simplex.vertices[k] = verts[i * nv + k];
@@ -292,6 +298,46 @@ class SimplicialEngine {
     }
     v.ptr = vec.data() + base;
     return v;
+  }
+
+  // Attribute recording (engine.hpp:1144-1201): the array, component and
+  // coordinates of each AttributeSpec are chosen as the reference does; the values
+  // are interpolated on the device.
+  void add_attributes(const std::map<std::string, ftk::ndarray<T>>& data, const RegularSimplicialMesh* reg,
+                      const ftk2h_mesh_desc& md) {
+    if constexpr (detail::pred_traits<PredicateType>::supported) {
+      if (predicate_.attributes.empty()) return;
+      const auto& gd = reg->get_global_dims();
+      const auto& off = reg->get_offset();
+      const int d = (int)gd.size();
+      std::vector<ftk2h_attribute_desc> ad;
+      for (const auto& spec : predicate_.attributes) {
+        if (spec.slot < 0 || spec.slot >= 16) continue;
+        auto it = data.find(spec.source);
+        if (it == data.end()) continue;
+        const auto& arr = it->second;
+        // a first extent of 1..16 marks a multi-component array (engine.hpp:1162)
+        const bool multi = arr.nd() >= 2 && arr.dimf(0) >= 1 && arr.dimf(0) <= 16;
+        ftk2h_attribute_desc a{};
+        a.slot = spec.slot;
+        if (multi && spec.component >= 0) {
+          if ((size_t)spec.component >= arr.dimf(0))
+            throw std::out_of_range("attribute component " + std::to_string(spec.component) + " of " + spec.source);
+          a.ncomp = 1;
+          a.comps[0] = component_view(arr, spec.component, gd, off, d);  // global coordinates
+        } else if (multi && spec.type == "magnitude") {
+          a.ncomp = (int)arr.dimf(0);
+          for (int c = 0; c < a.ncomp; ++c) a.comps[c] = component_view(arr, c, gd, off, d);
+        } else {
+          a.ncomp = 1;
+          a.comps[0] = legacy_view(arr, reg);  // get_value
+        }
+        ad.push_back(a);
+      }
+      if (!ad.empty())
+        detail::check(ftk2h_result_add_attributes(detail::context(device_), result_, &md, ad.data(), (int)ad.size(),
+                                                  nullptr));
+    }
   }
 
   std::shared_ptr<Mesh> mesh_;

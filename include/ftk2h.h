@@ -136,6 +136,25 @@ int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out);
 int ftk2h_result_phase_ms(const ftk2h_result* r, float* ms, int n);
 void ftk2h_result_free(ftk2h_result* r);
 
+/* ---- Attributes (engine.hpp:1144-1201, AttributeSpec predicate.hpp:34-41) --------
+ * FeatureElement::attributes[slot] = (float) sum_i bary_i * value_i over the node's
+ * vertices in ascending id order, bary_i the float barycentric coordinates, the sum
+ * in the predicate dtype.  value_i = comps[0] at the vertex (ncomp == 1) or the
+ * Euclidean norm of comps[0..ncomp) (ncomp 2..16, the reference's "magnitude").
+ * Views are indexed by local vertex coordinates of the extraction's mesh; the
+ * reference's choice of array, component and coordinates is the caller's (the C++
+ * engine in include/ftk2/core/engine.hpp makes it as the reference does).  Later
+ * entries overwrite earlier ones with the same slot. */
+typedef struct {
+  int slot;                      /* 0..15 */
+  int ncomp;                     /* 1..16 */
+  ftk2h_field_view comps[16];
+} ftk2h_attribute_desc;
+int ftk2h_result_add_attributes(ftk2h_ctx* ctx, ftk2h_result* r, const ftk2h_mesh_desc* mesh,
+                                const ftk2h_attribute_desc* attrs, int nattr, void* hip_stream);
+/* n_nodes * 16 floats (slots without an attribute are 0). */
+int ftk2h_result_copy_attributes(const ftk2h_result* r, float* out);
+
 /* ---- Streaming (SimplicialEngine::execute_stream, engine.hpp:214-270) ----------
  * Timesteps of a spatial grid arrive one at a time (ftk2h_stream_append, views
  * indexed by spatial coordinates).  Each append of timestep t >= 1 extracts the

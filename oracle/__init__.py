@@ -215,3 +215,40 @@ def encode_simplex_id(gdims, verts):
     g_a, g_p = _u64(list(gdims))
     v_a, v_p = _u64(list(verts))
     return int(lib().oracle_encode_simplex_id(d, g_p, len(verts) - 1, v_p))
+
+
+def attributes(verts, bary, specs, gdims, offset, dtype):
+    """Attribute values of FeatureElement::attributes (engine.hpp:1144-1201).
+
+    verts: (n, m+1) ascending global vertex ids of each node; bary: (n, m+1) float32
+    barycentric coordinates; specs: [(slot, [arrays indexed by local coordinates])]
+    in order (one array: its value; several: the Euclidean norm, the reference's
+    "magnitude"); gdims/offset: global extents and local offset of the mesh;
+    dtype: the predicate dtype, in which each sum sum_i bary_i * value_i runs term by
+    term from i = 0.  Returns (n, 16) float32, zero where no spec writes."""
+    T = np.dtype(dtype).type
+    verts = np.asarray(verts, dtype=np.uint64)
+    n, nv = verts.shape
+    d = len(gdims)
+    out = np.zeros((n, 16), np.float32)
+    # local coordinates of every vertex
+    coords = []
+    rest = verts.copy()
+    for j in range(d):
+        coords.append((rest % np.uint64(gdims[j])).astype(np.int64) - int(offset[j]))
+        rest = rest // np.uint64(gdims[j])
+    for slot, arrays in specs:
+        acc = np.zeros(n, dtype=T)
+        for k in range(nv):
+            idx = tuple(c[:, k] for c in coords)
+            if len(arrays) == 1:
+                v = np.asarray(arrays[0])[idx].astype(T)
+            else:
+                msq = np.zeros(n, dtype=T)
+                for a in arrays:
+                    x = np.asarray(a)[idx].astype(T)
+                    msq = (msq + x * x).astype(T)
+                v = np.sqrt(msq).astype(T)
+            acc = (acc + bary[:, k].astype(T) * v).astype(T)
+        out[:, slot] = acc.astype(np.float32)
+    return out

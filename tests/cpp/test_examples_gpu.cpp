@@ -158,7 +158,66 @@ static void streaming() {
   std::printf("streaming: %zu vertices streamed, %zu one-shot\n", streamed.vertices.size(), one.vertices.size());
 }
 
+// Attribute recording (engine.hpp:1144-1201) through the engine's AttributeSpec
+// rules: a scalar source, one component and the magnitude of a multi-component
+// array (read at global coordinates), and a coordinate field.  At critical points
+// of (U, V, W) the interpolated components and magnitude vanish; the coordinate
+// field interpolates to the feature position.
+static void attributes_3d() {
+  const int W = 20, H = 20, Dd = 20, Tn = 4;
+  ftk::ndarray<double> u({(size_t)W, (size_t)H, (size_t)Dd, (size_t)Tn}), v(u), w(u), xs(u);
+  ftk::ndarray<double> vec({3, (size_t)W, (size_t)H, (size_t)Dd, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t)
+    for (int z = 0; z < Dd; ++z)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+          const double a = x - 9.3 - 0.2 * t, b = y - 10.6 + 0.1 * t, c = z - 9.8;
+          u.f(x, y, z, t) = a + 0.1 * b;
+          v.f(x, y, z, t) = b - 0.05 * c;
+          w.f(x, y, z, t) = c + 0.02 * a;
+          xs.f(x, y, z, t) = x;
+          vec.f(0, x, y, z, t) = u.f(x, y, z, t);
+          vec.f(1, x, y, z, t) = v.f(x, y, z, t);
+          vec.f(2, x, y, z, t) = w.f(x, y, z, t);
+        }
+  std::map<std::string, ftk::ndarray<double>> data = {{"U", u}, {"V", v}, {"W", w}, {"X", xs}, {"VEC", vec}};
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{(uint64_t)W, (uint64_t)H, (uint64_t)Dd, (uint64_t)Tn});
+  CriticalPointPredicate<3, double> pred;
+  pred.var_names[0] = "U";
+  pred.var_names[1] = "V";
+  pred.var_names[2] = "W";
+  pred.use_multicomponent = false;
+  AttributeSpec s0; s0.source = "W"; s0.slot = 0;
+  AttributeSpec s1; s1.source = "VEC"; s1.type = "magnitude"; s1.slot = 1;
+  AttributeSpec s2; s2.source = "VEC"; s2.component = 0; s2.slot = 5;
+  AttributeSpec s3; s3.source = "X"; s3.slot = 9;
+  AttributeSpec s4; s4.source = "missing"; s4.slot = 2;  // skipped, as in the reference
+  pred.attributes = {s0, s1, s2, s3, s4};
+  SimplicialEngine<double, CriticalPointPredicate<3, double>> engine(mesh, pred);
+  engine.execute(data);
+  auto cx = engine.get_complex();
+  CHECK_EQ(cx.vertices.size() > 0, true);
+  for (const auto& el : cx.vertices) {
+    CHECK_EQ(std::fabs(el.attributes[0]) < 1e-5f, true);
+    double mag = 0;  // sum_i b_i |X_i| (the norm is taken per vertex, then interpolated)
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t id = el.simplex.vertices[k];
+      const int x = (int)(id % W), y = (int)(id / W % H), z = (int)(id / (W * H) % Dd), t = (int)(id / (W * H * Dd));
+      const double a = u.f(x, y, z, t), b = v.f(x, y, z, t), c = w.f(x, y, z, t);
+      mag += (double)el.barycentric_coords[0][k] * std::sqrt(a * a + b * b + c * c);
+    }
+    CHECK_EQ(std::fabs(el.attributes[1] - mag) <= 1e-6 * (1 + mag), true);
+    CHECK_EQ(std::fabs(el.attributes[5]) < 1e-5f, true);
+    CHECK_EQ(el.attributes[2] == 0.0f, true);
+    double px = 0;
+    for (int k = 0; k < 4; ++k) px += el.barycentric_coords[0][k] * (double)(el.simplex.vertices[k] % W);
+    CHECK_EQ(std::fabs(el.attributes[9] - px) < 1e-4, true);
+  }
+  std::printf("attributes_3d: %zu vertices\n", cx.vertices.size());
+}
+
 int main() {
+  attributes_3d();
   levelset_3d();
   fiber_3d();
   critical_point_3d();

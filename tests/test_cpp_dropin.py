@@ -39,3 +39,4 @@ def test_cpp_examples_on_gpu():
     assert p.returncode == 0, p.stdout + p.stderr
     assert "66418" in p.stdout and "6078" in p.stdout
     assert "streaming:" in p.stdout  # execute_stream == execute through ftk::stream<>
+    assert "attributes_3d:" in p.stdout
