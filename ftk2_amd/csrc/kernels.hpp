@@ -1380,12 +1380,15 @@ __device__ __forceinline__ void uf_unite(uint32_t* parent, uint32_t a, uint32_t 
   }
 }
 
+// Grid-stride over the *count pairs (the count is only known on the device, so the
+// grid is a fixed persistent size instead of one thread per possible pair).
 static __global__ __launch_bounds__(BLOCK) void k_link_pairs(uint32_t* parent, const uint32_t* pairs,
                                                       const uint32_t* count) {
-  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= *count) return;
-  uf_unite(parent, pairs[2 * i], pairs[2 * i + 1]);
+  const uint64_t n = *count;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+    uf_unite(parent, pairs[2 * i], pairs[2 * i + 1]);
 }
+constexpr unsigned LINK_GRID = 256 * 16;  // workgroups of the persistent link kernel
 
 static __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
