@@ -159,6 +159,7 @@ int run_locked(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** 
   if ((rc = S.mask.ensure(totv * 8))) return rc;
   if ((rc = S.pend.ensure(totv * 8))) return rc;
   if ((rc = S.degm.ensure(totv * 8))) return rc;
+  if ((rc = S.ccount.ensure(totv * 2))) return rc;
   if ((rc = S.node_off.ensure(totv * 4))) return rc;
   if ((rc = S.bsum.ensure((ntiles + 1) * 4))) return rc;
   if ((rc = S.ebsum.ensure((ntiles + 1) * 4))) return rc;
@@ -207,6 +208,7 @@ int run_locked(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** 
     P.mask = static_cast<uint64_t*>(S.mask.p) + vert_base[a];
     P.pend = static_cast<uint64_t*>(S.pend.p) + vert_base[a];
     P.degm = static_cast<uint64_t*>(S.degm.p) + vert_base[a];
+    P.ccount = static_cast<uint16_t*>(S.ccount.p) + vert_base[a];
     P.node_off = static_cast<uint32_t*>(S.node_off.p) + vert_base[a];
     P.bsum = static_cast<uint32_t*>(S.bsum.p) + tile_base[a];
     P.ebsum = static_cast<uint32_t*>(S.ebsum.p) + tile_base[a];
@@ -346,7 +348,7 @@ void ftk2h_batch_destroy(ftk2h_batch* b) {
   if (S.st) (void)hipStreamSynchronize(S.st);
   if (S.g1) (void)hipGraphExecDestroy(S.g1);
   if (S.g2) (void)hipGraphExecDestroy(S.g2);
-  for (DevBuf* p : {&S.jobs, &S.tile_base, &S.tile_job, &S.mask, &S.pend, &S.degm, &S.node_off, &S.bsum, &S.ebsum, &S.sums,
+  for (DevBuf* p : {&S.jobs, &S.tile_base, &S.tile_job, &S.mask, &S.pend, &S.degm, &S.ccount, &S.node_off, &S.bsum, &S.ebsum, &S.sums,
                     &S.counters, &S.bases, &S.xedges, &S.parent, &S.staging})
     p->release();
   if (S.host_bases) (void)hipHostFree(S.host_bases);

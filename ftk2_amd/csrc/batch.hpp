@@ -194,7 +194,7 @@ struct BatchState {
   hipStream_t st = nullptr;
   int n_cu = 256;
   BatchConfig cfg;
-  DevBuf jobs, tile_base, tile_job, mask, pend, degm, node_off, bsum, ebsum, sums, counters, bases, xedges, parent, staging;
+  DevBuf jobs, tile_base, tile_job, mask, pend, degm, ccount, node_off, bsum, ebsum, sums, counters, bases, xedges, parent, staging;
   uint32_t* host_bases = nullptr;  // pinned: node_base[njobs+1], cell_base[njobs+1]
   size_t host_bases_cap = 0;
   std::shared_ptr<Arena> arena;

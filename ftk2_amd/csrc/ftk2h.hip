@@ -135,6 +135,7 @@ void ftk2h_ctx_destroy(ftk2h_ctx* ctx) {
   ctx->mask.release();
   ctx->pend.release();
   ctx->degm.release();
+  ctx->ccount.release();
   ctx->node_off.release();
   ctx->bsum.release();
   ctx->ebsum.release();
@@ -223,6 +224,7 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   if ((rc = ctx->mask.ensure(nvert * 8))) return rc;
   if ((rc = ctx->pend.ensure(nvert * 8))) return rc;
   if ((rc = ctx->degm.ensure(nvert * 8))) return rc;
+  if ((rc = ctx->ccount.ensure(nvert * 2))) return rc;
   if ((rc = ctx->node_off.ensure(nvert * 4))) return rc;
   if ((rc = ctx->bsum.ensure((nb + 1) * 4))) return rc;
   if ((rc = ctx->ebsum.ensure((nb + 1) * 4))) return rc;
@@ -230,6 +232,7 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   P.mask = static_cast<uint64_t*>(ctx->mask.p);
   P.pend = static_cast<uint64_t*>(ctx->pend.p);
   P.degm = static_cast<uint64_t*>(ctx->degm.p);
+  P.ccount = static_cast<uint16_t*>(ctx->ccount.p);
   P.node_off = static_cast<uint32_t*>(ctx->node_off.p);
   P.bsum = static_cast<uint32_t*>(ctx->bsum.p);
   P.ebsum = static_cast<uint32_t*>(ctx->ebsum.p);

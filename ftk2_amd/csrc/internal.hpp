@@ -141,7 +141,7 @@ struct DevPool {
 struct ftk2h_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  ftk2h_internal::DevBuf mask, pend, degm, node_off, bsum, ebsum, xedges, xcount, scan_sums, fields[4];
+  ftk2h_internal::DevBuf mask, pend, degm, ccount, node_off, bsum, ebsum, xedges, xcount, scan_sums, fields[4];
   hipEvent_t ev[ftk2h_internal::NPHASE + 1] = {};
   std::shared_ptr<ftk2h_internal::DevPool> pool;
   ftk2h_internal::DevBuf parent;  // union-find scratch

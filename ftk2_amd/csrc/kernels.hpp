@@ -57,6 +57,7 @@ struct KParams {
   uint64_t* mask;
   uint64_t* pend;   // types left to stage 2 of the classification
   uint64_t* degm;   // those among them that need the verbatim sign levels
+  uint16_t* ccount; // manifold cells of the hypercube based at each vertex (k_offsets)
   uint32_t* node_off;
   uint32_t* bsum;   // per-block node counts -> exclusive offsets (+ total at [nb])
   uint32_t* ebsum;  // per-block cell counts  -> exclusive offsets (+ total at [nb])
@@ -1078,6 +1079,7 @@ __device__ __forceinline__ void offsets_tile(const KParams& P, int64_t blk) {
     x.v = v;
     decompose<D>(P, v, x.l);
     if (is_hc_base<D>(P, x.l)) nc = cells_of<KIND, D, M>(P, x, 0, (uint32_t*)nullptr);
+    if constexpr (CellKind<KIND, D, M>::verts > 0) gptr(P.ccount)[v] = (uint16_t)nc;
   }
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
@@ -1247,8 +1249,8 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   bool linked = false;
   // ---- cells
   if constexpr (NV > 0) {
-    uint32_t nc = 0;
-    if (hc) nc = cells_of<KIND, D, M>(P, x, 0, (uint32_t*)nullptr);
+    // the count k_offsets made for this hypercube
+    const uint32_t nc = (v < P.nvert && hc) ? (uint32_t)gptr(P.ccount)[v] : 0u;
     uint32_t ctot;
     const uint32_t ex = block_excl_scan(nc, &ctot);
     const uint64_t cbase = gptr(P.ebsum)[blk];
