@@ -315,6 +315,7 @@ constexpr TetOrder<D> TET_ORDER = build_tet_order<D>();
 
 struct Masks {
   uint64_t acc, sol, deg;
+  uint64_t tdeg = 0;  // degenerate simplices decidable from the stage-1 LDS tile (M = 3)
 };
 
 // LDS tile of quantized 3-component values.  A workgroup owns the 256 consecutive
@@ -418,7 +419,7 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
     deg |= (valid & z) << t;
     sol |= (valid & ~z & in & 1) << t;
   });
-  return Masks{acc, sol, deg};
+  return Masks{acc, sol, 0, deg};  // the verbatim levels run from the tile
 }
 
 // det2-based predicates on 2-simplices: CriticalPointPredicate<2,T> and
@@ -929,32 +930,7 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
 // Tile functions: one workgroup processes the 256 vertices [blk*256, blk*256 + 256)
 // of the mesh described by P.  The single-mesh kernels run one tile per workgroup;
 // the batched kernels (batch.hip) loop over tiles of many meshes.
-// Stage 1: every vertex's fast classification.  mask = accepted types, pend =
-// types that need stage 2 (SoS inside with the solver gate still to run, or the
-// verbatim sign levels), degm = the verbatim ones among them.
-template <int KIND, int D, int M, class T>
-__device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
-  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 16;
-  __shared__ __align__(16) float tile[TBYTES / 4];
-  const int64_t v0 = blk * BLOCK;
-  const int64_t v = v0 + threadIdx.x;
-  if constexpr (M == 3) {
-    stage_quantized3<T, D>(P, v0, tile);
-    __syncthreads();
-  }
-  if (v < P.nvert) {
-    const Vtx x = make_vtx<D>(P, v);
-    const Masks mk = classify_fast<KIND, D, M, T>(P, x, tile);
-    gptr(P.mask)[v] = mk.acc;
-    gptr(P.pend)[v] = mk.sol | mk.deg;
-    gptr(P.degm)[v] = mk.deg;
-  }
-}
-
-// Stage 2: the pending simplices of a tile, run from a workgroup queue in LDS (in
-// rounds of at most S2_QCAP items) so that every lane does useful work; then the
-// final mask and the tile's node count.
-constexpr int S2_QCAP = 2048;
+constexpr int S2_QCAP = 2048;  // queued items per round of a workgroup queue
 
 // Runs f(owner thread, type) over the items of every thread's bit set w (LDS
 // queue in rounds of at most S2_QCAP items, one item per lane).
@@ -978,6 +954,61 @@ __device__ __forceinline__ void run_items(uint16_t* queue, uint64_t w, F&& f) {
     __syncthreads();
   }
 }
+
+// Stage 1: every vertex's fast classification.  mask = accepted types, pend =
+// types that need stage 2 (SoS inside with the solver gate still to run, or the
+// verbatim sign levels), degm = the verbatim ones among them.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
+  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 16;
+  __shared__ __align__(16) float tile[TBYTES / 4];
+  const int64_t v0 = blk * BLOCK;
+  const int64_t v = v0 + threadIdx.x;
+  if constexpr (M == 3) {
+    stage_quantized3<T, D>(P, v0, tile);
+    __syncthreads();
+  }
+  Masks mk{0, 0, 0, 0};
+  if (v < P.nvert) {
+    const Vtx x = make_vtx<D>(P, v);
+    mk = classify_fast<KIND, D, M, T>(P, x, tile);
+  }
+  if constexpr (M == 3) {
+    // degenerate tets of vertices whose corners are all below 2^20: the verbatim
+    // sign levels on the exact quantized values of the tile (no gathers)
+    __shared__ uint16_t tq[S2_QCAP];
+    __shared__ unsigned long long prom[BLOCK];
+    __shared__ uint32_t pct[64];
+    if (threadIdx.x < 64) pct[threadIdx.x] = packed_chain<D, 3>(threadIdx.x);
+    prom[threadIdx.x] = 0;
+    run_items(tq, mk.tdeg, [&](int owner, int t) {
+      const uint32_t pc = pct[t];
+      double X[4][3];
+      double mx = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int s = k == 0 ? 0 : (int)((pc >> (4 * (k - 1))) & 0xF);
+        const float* src = tile + ((s >> 1) * Tile3<D>::SEG + owner + (s & 1)) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          X[k][c] = (double)src[c];
+          mx = fmax(mx, fabs(X[k][c]));
+        }
+      }
+      if (inside3_exact_d(X, mx)) atomicOr(&prom[owner], 1ull << t);
+    });
+    mk.sol |= prom[threadIdx.x];
+  }
+  if (v < P.nvert) {
+    gptr(P.mask)[v] = mk.acc;
+    gptr(P.pend)[v] = mk.sol | mk.deg;
+    gptr(P.degm)[v] = mk.deg;
+  }
+}
+
+// Stage 2: the pending simplices of a tile, run from a workgroup queue in LDS (in
+// rounds of at most S2_QCAP items) so that every lane does useful work; then the
+// final mask and the tile's node count.
 
 // Phase A runs the verbatim sign tests of the degenerate items; for critical
 // points the ones found inside join the solver items of phase B.  Each phase
