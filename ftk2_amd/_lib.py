@@ -46,7 +46,7 @@ class DeviceView(C.Structure):
     _fields_ = [("n_nodes", C.c_uint64), ("node_dim", C.c_int), ("keys", C.c_void_p),
                 ("lambda_", C.c_void_p), ("track_id", C.c_void_p), ("scalar", C.c_void_p),
                 ("n_conn", C.c_int), ("conn_dim", C.c_int * 3), ("conn_cells", C.c_uint64 * 3),
-                ("conn", C.c_void_p * 3)]
+                ("conn", C.c_void_p * 3), ("scalar_value", C.c_float)]
 
 
 class AttributeDesc(C.Structure):
@@ -72,6 +72,7 @@ SIGNATURES = {
     "ftk2h_result_sizes": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ftk2h_result_conn_info": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "ftk2h_result_copy_nodes": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "ftk2h_result_copy_node_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, _P, _P]),
     "ftk2h_result_copy_keys": (C.c_int, [_P, _P]),
     "ftk2h_result_copy_conn": (C.c_int, [_P, C.c_int, _P]),
     "ftk2h_result_device_view": (C.c_int, [_P, C.POINTER(DeviceView)]),

@@ -126,6 +126,7 @@ int run_locked(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** 
     R->dtype = S.pred.dtype;
     for (int k = 0; k < d; ++k) R->gdims[k] = J.mesh.global_dims[k];
     R->has_scalar = J.scalar && J.scalar->ptr && S.pred.kind == FTK2H_PRED_CRITICAL_POINT;
+    if (S.pred.kind == FTK2H_PRED_CONTOUR) R->scalar_value = (float)S.pred.thresholds[0];
     res[j] = std::move(R);
     if (has_cells) active.push_back(j);
   }

@@ -184,6 +184,8 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   res->m = m;
   res->dtype = pred->dtype;
   for (int j = 0; j < d; ++j) res->gdims[j] = mesh->global_dims[j];
+  // ContourPredicate::extract_it stores its threshold as the scalar (predicate.hpp:150)
+  if (pred->kind == FTK2H_PRED_CONTOUR) res->scalar_value = (float)pred->thresholds[0];
   if (!has_cells) {  // no hypercube: the reference visits nothing
     *out = res.release();
     return FTK2H_OK;
@@ -270,30 +272,35 @@ int ftk2h_result_copy_keys(const ftk2h_result* r, uint64_t* keys) {
   return FTK2H_OK;
 }
 
-int ftk2h_result_copy_nodes(const ftk2h_result* r, uint64_t* vertices, double* lambda, float* bary,
-                            float* scalar, uint64_t* track_id) {
+int ftk2h_result_copy_node_range(const ftk2h_result* r, uint64_t begin, uint64_t count,
+                                 uint64_t* vertices, double* lambda, float* bary, float* scalar,
+                                 uint64_t* track_id) {
   if (!r) return fail(FTK2H_E_INVALID, "null result");
-  const uint64_t n = r->n_nodes;
+  if (begin > r->n_nodes || count > r->n_nodes - begin)
+    return fail(FTK2H_E_INVALID, "node range beyond the result");
+  const uint64_t n = count;
   if (!n) return FTK2H_OK;
   HIP_TRY(hipSetDevice(r->device));
   const int nv = r->m + 1;
   if (vertices) {
     std::vector<uint64_t> keys(n);
-    HIP_TRY(hipMemcpy(keys.data(), r->keys, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(keys.data(), r->keys + begin, n * 8, hipMemcpyDeviceToHost));
     int rc = decode_keys(r, keys, vertices);
     if (rc) return rc;
   }
   if (lambda || bary) {
     if (r->dtype == FTK2H_F64) {
       std::vector<double> lam(n * nv);
-      HIP_TRY(hipMemcpy(lam.data(), r->lambda, n * nv * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(lam.data(), (const double*)r->lambda + begin * nv, n * nv * 8,
+                        hipMemcpyDeviceToHost));
       for (uint64_t i = 0; i < n * nv; ++i) {
         if (lambda) lambda[i] = lam[i];
         if (bary) bary[i] = (float)lam[i];
       }
     } else {
       std::vector<float> lam(n * nv);
-      HIP_TRY(hipMemcpy(lam.data(), r->lambda, n * nv * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(lam.data(), (const float*)r->lambda + begin * nv, n * nv * 4,
+                        hipMemcpyDeviceToHost));
       for (uint64_t i = 0; i < n * nv; ++i) {
         if (lambda) lambda[i] = (double)lam[i];
         if (bary) bary[i] = lam[i];
@@ -302,19 +309,23 @@ int ftk2h_result_copy_nodes(const ftk2h_result* r, uint64_t* vertices, double* l
   }
   if (scalar) {
     if (r->scalar) {
-      HIP_TRY(hipMemcpy(scalar, r->scalar, n * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(scalar, r->scalar + begin, n * 4, hipMemcpyDeviceToHost));
     } else {
-      // ContourPredicate stores its threshold; other predicates store 0 (filled
-      // on device for contour through the scalar buffer when present).
-      for (uint64_t i = 0; i < n; ++i) scalar[i] = 0.f;
+      for (uint64_t i = 0; i < n; ++i) scalar[i] = r->scalar_value;  // threshold (contour) or 0
     }
   }
   if (track_id) {
     std::vector<uint32_t> tr(n);
-    HIP_TRY(hipMemcpy(tr.data(), r->track, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tr.data(), r->track + begin, n * 4, hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < n; ++i) track_id[i] = tr[i];
   }
   return FTK2H_OK;
+}
+
+int ftk2h_result_copy_nodes(const ftk2h_result* r, uint64_t* vertices, double* lambda, float* bary,
+                            float* scalar, uint64_t* track_id) {
+  if (!r) return fail(FTK2H_E_INVALID, "null result");
+  return ftk2h_result_copy_node_range(r, 0, r->n_nodes, vertices, lambda, bary, scalar, track_id);
 }
 
 int ftk2h_result_copy_conn(const ftk2h_result* r, int i, uint32_t* indices) {
@@ -334,6 +345,7 @@ int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out) {
   out->lambda = r->lambda;
   out->track_id = r->track;
   out->scalar = r->scalar;
+  out->scalar_value = r->scalar_value;
   out->n_conn = r->n_conn;
   for (int i = 0; i < 3; ++i) {
     out->conn_dim[i] = r->conn_dim[i];

@@ -158,6 +158,7 @@ struct ftk2h_result {
   void* lambda = nullptr;
   uint32_t* track = nullptr;
   float* scalar = nullptr;
+  float scalar_value = 0.f;  // FeatureElement::scalar of every node when `scalar` is null
   float* attrs = nullptr;  // n_nodes * 16 (FeatureElement::attributes), when requested
   int n_conn = 0;
   int conn_dim[3] = {0, 0, 0};

@@ -401,6 +401,7 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
   for (int j = 0; j < S->sd; ++j) res->gdims[j] = S->spatial[j];
   res->gdims[S->sd] = S->t;
   res->ntypes = S->ntypes;
+  if (S->pred.kind == FTK2H_PRED_CONTOUR) res->scalar_value = (float)S->pred.thresholds[0];
   const uint64_t n = S->committed;
   res->n_nodes = n;
   if (n) {

@@ -188,6 +188,49 @@ class Result:
             _lib.check(_lib.lib().ftk2h_result_copy_attributes(self.h, out.ctypes.data))
         return out
 
+    def node_range(self, begin: int, count: int):
+        """(vertices, lambda, bary, scalar, track_id) of nodes [begin, begin+count)."""
+        n, m, _ = self.sizes()
+        nv = m + 1
+        verts = np.zeros((count, nv), np.uint64)
+        lam = np.zeros((count, nv), np.float64)
+        bary = np.zeros((count, nv), np.float32)
+        scal = np.zeros(count, np.float32)
+        track = np.zeros(count, np.uint64)
+        _lib.check(_lib.lib().ftk2h_result_copy_node_range(
+            self.h, begin, count, verts.ctypes.data, lam.ctypes.data, bary.ctypes.data,
+            scal.ctypes.data, track.ctypes.data))
+        return verts, lam, bary, scal, track
+
+    def track_ids(self):
+        n, _, _ = self.sizes()
+        track = np.zeros(n, np.uint64)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_nodes(self.h, None, None, None, None,
+                                                          track.ctypes.data))
+        return track
+
+    def lambdas(self):
+        n, m, _ = self.sizes()
+        lam = np.zeros((n, m + 1), np.float64)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_nodes(self.h, None, lam.ctypes.data, None, None,
+                                                          None))
+        return lam
+
+    def connectivity(self):
+        _, _, nconn = self.sizes()
+        conn = {}
+        for i in range(nconn):
+            dim = C.c_int()
+            cells = C.c_uint64()
+            _lib.check(_lib.lib().ftk2h_result_conn_info(self.h, i, C.byref(dim), C.byref(cells)))
+            buf = np.zeros(cells.value * (dim.value + 1), np.uint32)
+            if cells.value:
+                _lib.check(_lib.lib().ftk2h_result_copy_conn(self.h, i, buf.ctypes.data))
+            conn[dim.value] = buf.reshape(-1, dim.value + 1)
+        return conn
+
     def to_complex(self):
         n, m, nconn = self.sizes()
         nv = m + 1
@@ -200,16 +243,7 @@ class Result:
             _lib.check(_lib.lib().ftk2h_result_copy_nodes(self.h, verts.ctypes.data, lam.ctypes.data,
                                                           bary.ctypes.data, scal.ctypes.data,
                                                           track.ctypes.data))
-        conn = {}
-        for i in range(nconn):
-            dim = C.c_int()
-            cells = C.c_uint64()
-            _lib.check(_lib.lib().ftk2h_result_conn_info(self.h, i, C.byref(dim), C.byref(cells)))
-            buf = np.zeros(cells.value * (dim.value + 1), np.uint32)
-            if cells.value:
-                _lib.check(_lib.lib().ftk2h_result_copy_conn(self.h, i, buf.ctypes.data))
-            conn[dim.value] = buf.reshape(-1, dim.value + 1)
-        return FeatureComplex(verts, lam, bary, scal, track, conn)
+        return FeatureComplex(verts, lam, bary, scal, track, self.connectivity())
 
 
 @dataclass

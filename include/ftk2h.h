@@ -105,6 +105,8 @@ typedef struct {
   int conn_dim[3];
   uint64_t conn_cells[3];
   const uint32_t* conn[3];   /* conn_cells * (conn_dim+1) canonical indices, sorted */
+  float scalar_value;        /* FeatureElement::scalar of every node when scalar is NULL
+                                (the contour threshold, else 0) */
 } ftk2h_device_view;
 
 int ftk2h_ctx_create(int device, ftk2h_ctx** out);
@@ -128,6 +130,12 @@ int ftk2h_result_conn_info(const ftk2h_result* r, int i, int* dim, uint64_t* n_c
  * scalar: n floats; track_id: n (FeatureElement::track_id). */
 int ftk2h_result_copy_nodes(const ftk2h_result* r, uint64_t* vertices, double* lambda,
                             float* bary, float* scalar, uint64_t* track_id);
+/* The same for nodes [begin, begin+count) only (E_INVALID past the end): reads a
+ * sample of a large result without copying all of it.  No counterpart in the
+ * reference, whose get_complex() always copies every FeatureElement. */
+int ftk2h_result_copy_node_range(const ftk2h_result* r, uint64_t begin, uint64_t count,
+                                 uint64_t* vertices, double* lambda, float* bary, float* scalar,
+                                 uint64_t* track_id);
 int ftk2h_result_copy_keys(const ftk2h_result* r, uint64_t* keys);
 int ftk2h_result_copy_conn(const ftk2h_result* r, int i, uint32_t* indices);
 int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out);

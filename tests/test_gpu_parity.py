@@ -42,6 +42,7 @@ def compare(name, comps, kind, m, thr=(0.0, 0.0), offset=None, gdims=None, devic
         rel = np.max(np.abs(cx.lam - ref["lam"]) / den)
         assert rel <= tol, f"lambda rel diff {rel}"
     assert np.array_equal(cx.bary, ref["bary"]), "float barycentrics differ"
+    assert np.array_equal(cx.scalar, ref["scalar"]), "FeatureElement::scalar differs"
     assert np.array_equal(cx.track_id, ref["track"]), "track ids differ"
     assert sorted(cx.connectivity) == sorted(ref["conn"])
     for dim, cells in ref["conn"].items():
