@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "certify.hpp"
 #include "exact.hpp"
 #include "tables.hpp"
 
@@ -306,8 +307,8 @@ constexpr TetOrder<D> TET_ORDER = build_tet_order<D>();
 // vertex owns, decide from exact level-0 signs whether it is rejected, inside by
 // SoS (needs the solver gate for critical points), or degenerate (some level-0
 // value is exactly zero, or the magnitudes exceed the fast filter: needs the
-// verbatim multi-level sign tests).  Stage 2 runs those rare per-simplex jobs from
-// a workgroup queue in LDS so every lane does useful work (eval_item).
+// verbatim multi-level sign tests).  Stage 2 spreads those per-simplex jobs over
+// the workgroup's lanes (run_items) so every lane does useful work.
 //
 // Bits: acc = accepted, sol = SoS inside, solver pending, deg = verbatim pending.
 // A triple product whose filtered FP64 value is within its error bound (exact zeros
@@ -581,36 +582,37 @@ __device__ __forceinline__ bool item_solves(const KParams& P, const CornerTab& t
 }
 
 // ---------------------------------------------------------------------------
-// lambda of one node (type t at vertex l), written as T[M+1]; returns scalar.
+// Values of one node's vertices (type t owned by the vertex of r): vals[i][k] =
+// component k at vertex i (fiber and contour: minus the thresholds).
 template <int KIND, int D, int M, class T>
-__device__ __forceinline__ void node_lambda(const KParams& P, const CornerTab& tab, const VtxRef& r, int t,
-                                            T* lam, float* scalar) {
+__device__ __forceinline__ void node_values(const KParams& P, const CornerTab& tab, const VtxRef& r, int t,
+                                            T (*vals)[M]) {
   const uint32_t pc = tab.pc[t];
-  int off[M + 1];
-  off[0] = 0;
 #pragma unroll
-  for (int i = 1; i <= M; ++i) off[i] = (pc >> (4 * (i - 1))) & 0xF;
+  for (int i = 0; i <= M; ++i) {
+    const int off = i == 0 ? 0 : (int)((pc >> (4 * (i - 1))) & 0xF);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      T v = load_val_t<T>(P, tab, k, r, off);
+      if constexpr (KIND == KIND_FIBER || KIND == KIND_CONTOUR) v = v - (T)P.thr[k];
+      vals[i][k] = v;
+    }
+  }
+}
+
+// lambda of one node from its vertex values, written as T[M+1]; returns scalar.
+template <int KIND, int D, int M, class T>
+__device__ __forceinline__ void node_solve(const KParams& P, const CornerTab& tab, const VtxRef& r, int t,
+                                           const T (*vals)[M], T* lam, float* scalar) {
   *scalar = 0.f;
   if constexpr (M == 1) {
-    T v0 = load_val_t<T>(P, tab, 0, r, off[0]);
-    T v1 = load_val_t<T>(P, tab, 0, r, off[1]);
     if constexpr (KIND == KIND_CONTOUR) {
-      v0 = v0 - (T)P.thr[0];
-      v1 = v1 - (T)P.thr[0];
-      if (!solve_1<T>(v0, v1, lam)) lam[0] = lam[1] = (T)0.5;
+      if (!solve_1<T>(vals[0][0], vals[1][0], lam)) lam[0] = lam[1] = (T)0.5;
       *scalar = (float)(T)P.thr[0];
     } else {
-      solve_1<T>(v0, v1, lam);
+      solve_1<T>(vals[0][0], vals[1][0], lam);
     }
   } else {
-    T vals[M + 1][M];
-#pragma unroll
-    for (int i = 0; i <= M; ++i)
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        vals[i][k] = load_val_t<T>(P, tab, k, r, off[i]);
-        if constexpr (KIND == KIND_FIBER) vals[i][k] = vals[i][k] - (T)P.thr[k];
-      }
     const bool ok = solve_k<M, T>(vals, lam);
     if constexpr (KIND == KIND_FIBER) {
       if (!ok)
@@ -620,9 +622,13 @@ __device__ __forceinline__ void node_lambda(const KParams& P, const CornerTab& t
   }
   if constexpr (KIND == KIND_CP) {
     if (P.scal) {  // predicate.hpp:81-92
+      const uint32_t pc = tab.pc[t];
       T sv = T(0);
 #pragma unroll
-      for (int i = 0; i <= M; ++i) sv = sv + lam[i] * gptr(static_cast<const T*>(P.scal))[r.sbase + tab.soff[off[i]]];
+      for (int i = 0; i <= M; ++i) {
+        const int off = i == 0 ? 0 : (int)((pc >> (4 * (i - 1))) & 0xF);
+        sv = sv + lam[i] * gptr(static_cast<const T*>(P.scal))[r.sbase + tab.soff[off]];
+      }
       *scalar = (float)sv;
     }
   }
@@ -930,29 +936,74 @@ __device__ __forceinline__ uint32_t cells_of(const KParams& P, const Vtx& x, int
 // Tile functions: one workgroup processes the 256 vertices [blk*256, blk*256 + 256)
 // of the mesh described by P.  The single-mesh kernels run one tile per workgroup;
 // the batched kernels (batch.hip) loop over tiles of many meshes.
-constexpr int S2_QCAP = 2048;  // queued items per round of a workgroup queue
-
-// Runs f(owner thread, type) over the items of every thread's bit set w (LDS
-// queue in rounds of at most S2_QCAP items, one item per lane).
-template <class F>
-__device__ __forceinline__ void run_items(uint16_t* queue, uint64_t w, F&& f) {
-  while (true) {
-    uint32_t tot;
-    uint32_t pos = block_excl_scan((uint32_t)__popcll(w), &tot);
-    if (tot == 0) break;
-    while (w && pos < (uint32_t)S2_QCAP) {
-      const int t = __builtin_ctzll(w);
-      w &= w - 1;
-      queue[pos++] = (uint16_t)(threadIdx.x | (t << 8));
-    }
-    __syncthreads();
-    const uint32_t n = tot < (uint32_t)S2_QCAP ? tot : (uint32_t)S2_QCAP;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-      const uint32_t it = queue[i];
-      f((int)(it & 0xFF), (int)(it >> 8));
-    }
-    __syncthreads();
+// Position of the n-th set bit of m (n < popcount(m)), by halving.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, uint32_t n) {
+  int pos = 0;
+  uint32_t x = (uint32_t)m;
+  uint32_t c = __popc(x);
+  if (n >= c) {
+    n -= c;
+    x = (uint32_t)(m >> 32);
+    pos = 32;
   }
+#pragma unroll
+  for (int h = 16; h >= 1; h >>= 1) {
+    c = __popc(x & ((1u << h) - 1u));
+    if (n >= c) {
+      n -= c;
+      x >>= h;
+      pos += h;
+    }
+  }
+  return pos;
+}
+
+// The items of a tile: thread l owns the set bits of its mask w_l.  Items are
+// numbered in (thread, bit) order; item j belongs to the thread l with
+// offs[l] <= j < offs[l+1] (offs = exclusive scan of the popcounts) and is that
+// thread's (j - offs[l])-th set bit.  Consecutive threads take consecutive items,
+// so every lane does useful work however the bits are spread, without building a
+// queue.
+struct ItemLds {
+  uint32_t offs[BLOCK + 1];
+  unsigned long long masks[BLOCK];
+};
+
+__device__ __forceinline__ void item_at(const ItemLds& L, uint32_t j, int& owner, int& t) {
+  int lo = 0, hi = BLOCK;  // offs[lo] <= j < offs[hi]
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {  // BLOCK = 256
+    const int mid = (lo + hi) >> 1;
+    if (L.offs[mid] <= j)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  owner = lo;
+  t = nth_set_bit(L.masks[lo], j - L.offs[lo]);
+}
+
+// Publishes every thread's w in L; returns the item count.  Ends with a barrier.
+__device__ __forceinline__ uint32_t publish_items(ItemLds& L, uint64_t w) {
+  uint32_t tot;
+  const uint32_t o = block_excl_scan((uint32_t)__popcll(w), &tot);
+  L.offs[threadIdx.x] = o;
+  L.masks[threadIdx.x] = w;
+  if (threadIdx.x == 0) L.offs[BLOCK] = tot;
+  __syncthreads();
+  return tot;
+}
+
+// Runs f(owner thread, type) over the items of every thread's bit set w.
+template <class F>
+__device__ __forceinline__ void run_items(ItemLds& L, uint64_t w, F&& f) {
+  const uint32_t tot = publish_items(L, w);
+  for (uint32_t j = threadIdx.x; j < tot; j += BLOCK) {
+    int owner, t;
+    item_at(L, j, owner, t);
+    f(owner, t);
+  }
+  __syncthreads();
 }
 
 // Stage 1: every vertex's fast classification.  mask = accepted types, pend =
@@ -976,12 +1027,12 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
   if constexpr (M == 3) {
     // degenerate tets of vertices whose corners are all below 2^20: the verbatim
     // sign levels on the exact quantized values of the tile (no gathers)
-    __shared__ uint16_t tq[S2_QCAP];
+    __shared__ ItemLds il;
     __shared__ unsigned long long prom[BLOCK];
     __shared__ uint32_t pct[64];
     if (threadIdx.x < 64) pct[threadIdx.x] = packed_chain<D, 3>(threadIdx.x);
     prom[threadIdx.x] = 0;
-    run_items(tq, mk.tdeg, [&](int owner, int t) {
+    run_items(il, mk.tdeg, [&](int owner, int t) {
       const uint32_t pc = pct[t];
       double X[4][3];
       double mx = 0;
@@ -1006,16 +1057,17 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
   }
 }
 
-// Stage 2: the pending simplices of a tile, run from a workgroup queue in LDS (in
-// rounds of at most S2_QCAP items) so that every lane does useful work; then the
-// final mask and the tile's node count.
+// Stage 2: the pending simplices of a tile, spread over the workgroup's lanes by
+// run_items so that every lane does useful work; then the final mask and the
+// tile's node count.
 
 // Phase A runs the verbatim sign tests of the degenerate items; for critical
-// points the ones found inside join the solver items of phase B.  Each phase
+// points the ones found inside join the solver items of phase B (B1: the solver
+// certificate for FP64 values, B2: the solver for what B1 leaves).  Each phase
 // keeps a wave on one code path.
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
-  __shared__ uint16_t queue[S2_QCAP];
+  __shared__ ItemLds il;
   __shared__ unsigned long long slot[BLOCK];
   __shared__ unsigned long long prom[BLOCK];
   __shared__ VtxRef refs[BLOCK];
@@ -1033,7 +1085,7 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
     slot[threadIdx.x] = acc;
     prom[threadIdx.x] = 0;
     if constexpr (M >= 2) {
-      run_items(queue, w & deg, [&](int owner, int t) {
+      run_items(il, w & deg, [&](int owner, int t) {
         if (item_inside<KIND, D, M, T>(P, tab, refs[owner], t)) {
           if constexpr (KIND == KIND_FIBER)
             atomicOr(&slot[owner], 1ull << t);
@@ -1043,8 +1095,24 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
       });
     }
     if constexpr (KIND != KIND_FIBER) {
-      const uint64_t ws = (w & ~deg) | prom[threadIdx.x];
-      run_items(queue, ws, [&](int owner, int t) {
+      uint64_t ws = (w & ~deg) | prom[threadIdx.x];
+      if constexpr (KIND == KIND_CP && M >= 2 && std::is_same<T, double>::value) {
+        // Phase B1: simplices whose solve is certified to succeed are accepted
+        // without running the solver; the others go on to phase B2.  Each thread
+        // only touches its own prom entry before the first barrier of run_items.
+        prom[threadIdx.x] = 0;
+        run_items(il, ws, [&](int owner, int t) {
+          T vals[M + 1][M];
+          item_values<KIND, D, M, T>(P, tab, refs[owner], t, vals);
+          if (solver_certified<M>(vals))
+            atomicOr(&slot[owner], 1ull << t);
+          else
+            atomicOr(&prom[owner], 1ull << t);
+        });
+        ws = prom[threadIdx.x];
+      }
+      // Phase B2: the solver gate itself
+      run_items(il, ws, [&](int owner, int t) {
         if (item_solves<KIND, D, M, T>(P, tab, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
       });
     }
@@ -1185,9 +1253,10 @@ __device__ __forceinline__ uint32_t cell_rank(const uint32_t* stage, uint32_t s,
 }
 
 // Emit: the nodes of a workgroup occupy one contiguous output range in canonical
-// order (vertex, then type), so they are queued in LDS in that order and written by
-// consecutive threads (coalesced, load balanced).  Cells are staged in LDS, each
-// vertex's segment sorted, and copied out contiguously.
+// order (vertex, then type), so they are queued in LDS in that order and node j
+// of the workgroup is solved and written by thread j mod 256 (coalesced, load
+// balanced).  Cells are staged in LDS, each vertex's segment sorted, and copied
+// out contiguously.
 constexpr int EMIT_NQ = 6144;          // queued nodes per workgroup
 constexpr int EMIT_LUF = EMIT_NQ / 2;   // nodes of the workgroup union-find (u32 in nq)
 
@@ -1224,7 +1293,7 @@ template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   constexpr int NTY = Chains<D, M>::N;
   constexpr int NV = CellKind<KIND, D, M>::verts;
-  __shared__ uint16_t nq[EMIT_NQ];
+  __shared__ uint16_t nq[EMIT_NQ];  // node queue, then the workgroup union-find
   __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t stage[EMIT_STAGE_WORDS];
   __shared__ CornerTab tab;
@@ -1244,10 +1313,10 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   const uint32_t nbase = gptr(P.bsum)[blk];
   const uint32_t ntot = gptr(P.bsum)[blk + 1] - nbase;
   auto lam_out = gptr(static_cast<T*>(P.lambda));
-  auto emit_node = [&](uint32_t pos, const VtxRef& r, int t) {
+  auto emit_from = [&](uint32_t pos, const VtxRef& r, int t, const T (*vals)[M]) {
     T lam[M + 1];
     float sc;
-    node_lambda<KIND, D, M, T>(P, tab, r, t, lam, &sc);
+    node_solve<KIND, D, M, T>(P, tab, r, t, vals, lam, &sc);
     gptr(P.keys)[pos] = r.gid * (uint64_t)NTY + (uint64_t)t;
 #pragma unroll
     for (int i = 0; i <= M; ++i) lam_out[(uint64_t)pos * (M + 1) + i] = lam[i];
@@ -1256,25 +1325,40 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   if (ntot <= P.nq_cap) {
     if (m) {
       uint32_t pos = (v < P.nvert ? gptr(P.node_off)[v] : 0) - nbase;
-      uint64_t mm = m;
-      while (mm) {
-        const int t = __builtin_ctzll(mm);
-        mm &= mm - 1;
-        nq[pos++] = (uint16_t)(threadIdx.x | (t << 8));
-      }
+      for (uint64_t mm = m; mm; mm &= mm - 1) nq[pos++] = (uint16_t)(threadIdx.x | (__builtin_ctzll(mm) << 8));
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) {
-      const uint32_t it = nq[i];
-      emit_node(nbase + i, refs[it & 0xFF], (int)(it >> 8));
+    // software pipelined: the next node's gathers are in flight while this one is solved
+    uint32_t i = threadIdx.x;
+    if (i < ntot) {
+      uint32_t it = nq[i];
+      T cur[M + 1][M];
+      node_values<KIND, D, M, T>(P, tab, refs[it & 0xFF], (int)(it >> 8), cur);
+      while (true) {
+        const uint32_t j = i + BLOCK;
+        uint32_t jt = 0;
+        T nxt[M + 1][M];
+        if (j < ntot) {
+          jt = nq[j];
+          node_values<KIND, D, M, T>(P, tab, refs[jt & 0xFF], (int)(jt >> 8), nxt);
+        }
+        emit_from(nbase + i, refs[it & 0xFF], (int)(it >> 8), cur);
+        if (j >= ntot) break;
+        i = j;
+        it = jt;
+#pragma unroll
+        for (int a = 0; a <= M; ++a)
+#pragma unroll
+          for (int k = 0; k < M; ++k) cur[a][k] = nxt[a][k];
+      }
     }
-  } else if (m) {
+  } else if (m) {  // more nodes than the queue holds: each thread writes its own
     uint32_t pos = gptr(P.node_off)[v];
-    uint64_t mm = m;
-    while (mm) {
+    for (uint64_t mm = m; mm; mm &= mm - 1) {
       const int t = __builtin_ctzll(mm);
-      mm &= mm - 1;
-      emit_node(pos++, x.r, t);
+      T vals[M + 1][M];
+      node_values<KIND, D, M, T>(P, tab, x.r, t, vals);
+      emit_from(pos++, x.r, t, vals);
     }
   }
   bool linked = false;
