@@ -202,6 +202,15 @@ class Result:
             scal.ctypes.data, track.ctypes.data))
         return verts, lam, bary, scal, track
 
+    def positions(self):
+        """(n, 3) float32 node positions as the reference's VTK writer computes them
+        (computed on the device)."""
+        n, _, _ = self.sizes()
+        xyz = np.zeros((n, 3), np.float32)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_positions(self.h, xyz.ctypes.data))
+        return xyz
+
     def track_ids(self):
         n, _, _ = self.sizes()
         track = np.zeros(n, np.uint64)

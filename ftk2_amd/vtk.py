@@ -1,0 +1,201 @@
+"""FeatureComplex -> VTK XML files (.vtu / .vtp) without the VTK library, with the
+arrays and cells of the reference writer (hguo/ftk2 include/ftk2/utils/vtk.hpp:26-160;
+the C++ twin is include/ftk2/utils/vtk.hpp here):
+
+  points   float32  sum_i (double)bary_i * vertex coordinates (global grid indices),
+                    first three coordinates (vtkPoints stores float)
+  Scalar   float32  FeatureElement::scalar
+  TrackID  int32    (int)track_id
+  Type     int32    FeatureElement::type (0 for every predicate on this path)
+  Time     float32  0 (a RegularSimplicialMesh has as many spatial as total dims)
+  .vtu     target_dim 0: one VTK_VERTEX per node, else connectivity dims 0..3
+  .vtp     verts / lines / polys from connectivity dims 0 / 1 / 2
+
+Data are inline ASCII (ascii=True, or FTK_DEBUG set) or appended and base64
+encoded without compression (header_type UInt64), the reference's two modes.
+`read()` parses files in both modes (the tests use it).
+"""
+from __future__ import annotations
+
+import base64
+import os
+import re
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+VTK_TYPE = {0: 1, 1: 3, 2: 5, 3: 10}  # VERTEX, LINE, TRIANGLE, TETRA
+_NP = {"Float32": np.float32, "Float64": np.float64, "Int32": np.int32, "Int64": np.int64, "UInt8": np.uint8}
+
+
+def positions(vertices, bary, global_dims):
+    """Reference positions of nodes given their sorted vertex ids (n, m+1) and
+    float barycentric coordinates (n, m+1): (n, 3) float32."""
+    vertices = np.asarray(vertices, dtype=np.uint64)
+    n, nv = vertices.shape
+    gd = [int(g) for g in global_dims]
+    pos = np.zeros((n, len(gd)), np.float64)
+    for i in range(nv):
+        rest = vertices[:, i].copy()
+        b = np.asarray(bary[:, i], dtype=np.float32).astype(np.float64)
+        for j, g in enumerate(gd):
+            c = (rest % np.uint64(g)).astype(np.float64)
+            rest //= np.uint64(g)
+            pos[:, j] = pos[:, j] + b * c
+    out = np.zeros((n, 3), np.float32)
+    k = min(3, len(gd))
+    out[:, :k] = pos[:, :k].astype(np.float32)
+    return out
+
+
+class _Writer:
+    def __init__(self, ascii_mode):
+        self.ascii = ascii_mode
+        self.appended = []
+        self.offset = 0
+
+    def array(self, lines, vtype, name, arr, ncomp=1):
+        arr = np.ascontiguousarray(arr, dtype=_NP[vtype])
+        head = f'        <DataArray type="{vtype}" Name="{name}"'
+        if ncomp > 1:
+            head += f' NumberOfComponents="{ncomp}"'
+        if self.ascii:
+            flat = arr.ravel()
+            text = " ".join(repr(x) for x in flat.tolist())  # round-trips every value
+            lines.append(head + ' format="ascii">')
+            lines.append("          " + text)
+            lines.append("        </DataArray>")
+        else:
+            lines.append(head + f' format="appended" offset="{self.offset}"/>')
+            raw = arr.tobytes()
+            chunk = base64.b64encode(np.uint64(len(raw)).tobytes()) + base64.b64encode(raw)
+            self.appended.append(chunk)
+            self.offset += len(chunk)
+
+    def finish(self, lines):
+        if not self.ascii:
+            lines.append('  <AppendedData encoding="base64">')
+            lines.append("   _" + b"".join(self.appended).decode("ascii"))
+            lines.append("  </AppendedData>")
+        lines.append("</VTKFile>")
+
+
+def _mode(ascii_mode):
+    return bool(ascii_mode) or os.environ.get("FTK_DEBUG") is not None
+
+
+def _point_section(lines, w, points, scalar, track, ntype):
+    n = len(points)
+    lines.append("      <PointData>")
+    w.array(lines, "Float32", "Scalar", scalar)
+    w.array(lines, "Int32", "TrackID", np.asarray(track).astype(np.int64).astype(np.int32))
+    w.array(lines, "Int32", "Type", ntype)
+    w.array(lines, "Float32", "Time", np.zeros(n, np.float32))
+    lines.append("      </PointData>")
+    lines.append("      <Points>")
+    w.array(lines, "Float32", "Points", points, 3)
+    lines.append("      </Points>")
+
+
+def _inputs(cx, global_dims, points):
+    if points is None:
+        points = positions(cx.vertices, cx.bary, global_dims)
+    n = len(cx.vertices)
+    return points, np.asarray(cx.scalar, np.float32), np.asarray(cx.track_id), np.zeros(n, np.int32)
+
+
+def write_vtu(cx, global_dims, filename, target_dim=-1, ascii=False, points=None):
+    """write_complex_to_vtu (utils/vtk.hpp:118-160).  cx: ftk2_amd.engine.FeatureComplex;
+    points: positions computed elsewhere (e.g. Result.positions()), else computed here."""
+    points, scalar, track, ntype = _inputs(cx, global_dims, points)
+    n = len(points)
+    conn, offs, types = [], [], []
+    if target_dim == 0:
+        conn = [np.arange(n, dtype=np.int64)]
+        offs = [np.arange(1, n + 1, dtype=np.int64)]
+        types = [np.full(n, 1, np.uint8)]
+    else:
+        base = 0
+        for dim in sorted(cx.connectivity):
+            if (target_dim != -1 and dim != target_dim) or dim not in VTK_TYPE:
+                continue
+            c = np.asarray(cx.connectivity[dim], dtype=np.int64).reshape(-1, dim + 1)
+            conn.append(c.ravel())
+            offs.append(base + (dim + 1) * np.arange(1, len(c) + 1, dtype=np.int64))
+            base += c.size
+            types.append(np.full(len(c), VTK_TYPE[dim], np.uint8))
+    cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
+    conn, offs, types = cat(conn, np.int64), cat(offs, np.int64), cat(types, np.uint8)
+    w = _Writer(_mode(ascii))
+    lines = ['<?xml version="1.0"?>',
+             '<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian" header_type="UInt64">',
+             "  <UnstructuredGrid>", f'    <Piece NumberOfPoints="{n}" NumberOfCells="{len(types)}">']
+    _point_section(lines, w, points, scalar, track, ntype)
+    lines.append("      <Cells>")
+    w.array(lines, "Int64", "connectivity", conn)
+    w.array(lines, "Int64", "offsets", offs)
+    w.array(lines, "UInt8", "types", types)
+    lines += ["      </Cells>", "    </Piece>", "  </UnstructuredGrid>"]
+    w.finish(lines)
+    with open(filename, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_vtp(cx, global_dims, filename, target_dim=-1, ascii=False, points=None):
+    """write_complex_to_vtp (utils/vtk.hpp:78-113)."""
+    points, scalar, track, ntype = _inputs(cx, global_dims, points)
+    n = len(points)
+    sect = {}
+    for dim in (0, 1, 2):
+        c = cx.connectivity.get(dim)
+        if c is None or (target_dim != -1 and dim != target_dim):
+            c = np.zeros((0, dim + 1), np.int64)
+        c = np.asarray(c, dtype=np.int64).reshape(-1, dim + 1)
+        sect[dim] = (c.ravel(), (dim + 1) * np.arange(1, len(c) + 1, dtype=np.int64))
+    w = _Writer(_mode(ascii))
+    lines = ['<?xml version="1.0"?>',
+             '<VTKFile type="PolyData" version="1.0" byte_order="LittleEndian" header_type="UInt64">',
+             "  <PolyData>",
+             f'    <Piece NumberOfPoints="{n}" NumberOfVerts="{len(sect[0][1])}" NumberOfLines="{len(sect[1][1])}" '
+             f'NumberOfStrips="0" NumberOfPolys="{len(sect[2][1])}">']
+    _point_section(lines, w, points, scalar, track, ntype)
+    for dim, name in ((0, "Verts"), (1, "Lines"), (2, "Polys")):
+        lines.append(f"      <{name}>")
+        w.array(lines, "Int64", "connectivity", sect[dim][0])
+        w.array(lines, "Int64", "offsets", sect[dim][1])
+        lines.append(f"      </{name}>")
+    lines += ["    </Piece>", "  </PolyData>"]
+    w.finish(lines)
+    with open(filename, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read(filename):
+    """Parse a .vtu/.vtp written in either mode: {"type", "piece": attrs,
+    "<Section>/<Name>": array}."""
+    text = open(filename).read()
+    appended = None
+    m = re.search(r'<AppendedData encoding="base64">\s*_(\S*)\s*</AppendedData>', text)
+    if m:
+        appended = m.group(1)
+        text = text[:m.start()] + text[m.end():]
+    root = ET.fromstring(text)
+    kind = root.get("type")
+    piece = root.find(f"{kind}/Piece")
+    out = {"type": kind, "piece": dict(piece.attrib)}
+    for sec in piece:
+        for da in sec.findall("DataArray"):
+            dt = _NP[da.get("type")]
+            ncomp = int(da.get("NumberOfComponents", "1"))
+            if da.get("format") == "ascii":
+                vals = (da.text or "").split()
+                arr = np.array([float(v) for v in vals], dtype=np.float64).astype(dt) if vals else np.zeros(0, dt)
+            else:
+                off = int(da.get("offset"))
+                nbytes = int(np.frombuffer(base64.b64decode(appended[off:off + 12]), np.uint64)[0])
+                nchar = 4 * ((nbytes + 2) // 3)
+                arr = np.frombuffer(base64.b64decode(appended[off + 12:off + 12 + nchar]), dtype=dt)
+            if ncomp > 1:
+                arr = arr.reshape(-1, ncomp)
+            out[f"{sec.tag}/{da.get('Name')}"] = arr
+    return out

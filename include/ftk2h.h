@@ -163,6 +163,12 @@ int ftk2h_result_add_attributes(ftk2h_ctx* ctx, ftk2h_result* r, const ftk2h_mes
 /* n_nodes * 16 floats (slots without an attribute are 0). */
 int ftk2h_result_copy_attributes(const ftk2h_result* r, float* out);
 
+/* Feature positions as the reference's VTK writer computes them
+ * (utils/vtk.hpp:26-49): xyz[3*i + j] = (float) sum_k (double)bary_k * coord_j of
+ * the node's vertices (global grid indices), first three coordinates, 0 beyond
+ * the mesh dimension.  Host buffer of 3 * n_nodes floats. */
+int ftk2h_result_copy_positions(const ftk2h_result* r, float* xyz);
+
 /* ---- Streaming (SimplicialEngine::execute_stream, engine.hpp:214-270) ----------
  * Timesteps of a spatial grid arrive one at a time (ftk2h_stream_append, views
  * indexed by spatial coordinates).  Each append of timestep t >= 1 extracts the
