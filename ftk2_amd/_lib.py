@@ -26,7 +26,7 @@ class Ftk2hError(RuntimeError):
 
 class MeshDesc(C.Structure):
     _fields_ = [("ndims", C.c_int), ("local_dims", C.c_uint64 * 4), ("offset", C.c_uint64 * 4),
-                ("global_dims", C.c_uint64 * 4)]
+                ("global_dims", C.c_uint64 * 4), ("time_id_stride", C.c_uint64)]
 
 
 class PredicateDesc(C.Structure):
@@ -72,7 +72,7 @@ SIGNATURES = {
     "ftk2h_result_sizes": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ftk2h_result_conn_info": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "ftk2h_result_copy_nodes": (C.c_int, [_P, _P, _P, _P, _P, _P]),
-    "ftk2h_result_copy_positions": (C.c_int, [_P, _P]),
+    "ftk2h_result_copy_positions": (C.c_int, [_P, _P, _P]),
     "ftk2h_result_copy_node_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, _P, _P]),
     "ftk2h_result_copy_keys": (C.c_int, [_P, _P]),
     "ftk2h_result_copy_conn": (C.c_int, [_P, C.c_int, _P]),

@@ -24,6 +24,7 @@ struct AttrDev {
 
 struct AttrParams {
   uint64_t gdims[4];
+  uint64_t tstride;  // id step of the last axis when extruded, else 0
   int64_t off[4];
   uint64_t n;
   int ntypes, nattr;
@@ -39,7 +40,13 @@ __global__ __launch_bounds__(BLOCK) void k_attributes(AttrParams A, const uint64
   uint64_t r = key / (uint64_t)A.ntypes;
   const int t = (int)(key - r * (uint64_t)A.ntypes);
   int64_t l[4] = {0, 0, 0, 0};
-  for (int j = 0; j < D; ++j) {
+  int sd = D;
+  if (A.tstride) {  // extruded: layer first, then the spatial id
+    l[D - 1] = (int64_t)(r / A.tstride) - A.off[D - 1];
+    r %= A.tstride;
+    sd = D - 1;
+  }
+  for (int j = 0; j < sd; ++j) {
     const uint64_t g = r % A.gdims[j];
     r /= A.gdims[j];
     l[j] = (int64_t)g - A.off[j];
@@ -108,7 +115,8 @@ int ftk2h_result_add_attributes(ftk2h_ctx* ctx, ftk2h_result* R, const ftk2h_mes
                                 const ftk2h_attribute_desc* attrs, int nattr, void* hip_stream) {
   if (!ctx || !R || !mesh || (nattr > 0 && !attrs)) return fail(FTK2H_E_INVALID, "null argument");
   if (nattr < 0 || nattr > 64) return fail(FTK2H_E_INVALID, "nattr must be 0..64");
-  if (mesh->ndims != R->d) return fail(FTK2H_E_INVALID, "mesh differs from the result's");
+  if (mesh->ndims != R->d || mesh->time_id_stride != R->tstride)
+    return fail(FTK2H_E_INVALID, "mesh differs from the result's");
   uint64_t nvert = 0;
   bool has_cells = false;
   if (int rc = check_mesh(mesh, R->d, &nvert, &has_cells)) return rc;
@@ -178,6 +186,7 @@ int ftk2h_result_add_attributes(ftk2h_ctx* ctx, ftk2h_result* R, const ftk2h_mes
     A.gdims[j] = j < R->d ? R->gdims[j] : 1;
     A.off[j] = j < R->d ? (int64_t)mesh->offset[j] : 0;
   }
+  A.tstride = R->tstride;
   A.n = n;
   A.ntypes = R->ntypes;
   A.nattr = nattr;

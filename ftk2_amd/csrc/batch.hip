@@ -118,6 +118,8 @@ int run_locked(ftk2h_batch* b, int njobs, const ftk2h_job* jobs, ftk2h_result** 
     bool has_cells = false;
     if (int rc = check_mesh(&J.mesh, d, &nverts[j], &has_cells))
       return fail(rc, "job " + std::to_string(j) + ": " + error_slot());
+    if (J.mesh.time_id_stride)
+      return fail(FTK2H_E_UNSUPPORTED, "job " + std::to_string(j) + ": batches take regular meshes only");
     auto R = std::make_unique<ftk2h_result>();
     R->device = S.ctx->device;
     R->kind = S.pred.kind;

@@ -78,6 +78,7 @@ void decode_keys_t(const ftk2h_result* r, const std::vector<uint64_t>& keys, uin
     gs[j] = s;
     s *= r->gdims[j];
   }
+  if (r->tstride) gs[D - 1] = r->tstride;  // extruded layers
   for (size_t i = 0; i < keys.size(); ++i) {
     const uint64_t vmin = keys[i] / (uint64_t)C.n;
     const int t = (int)(keys[i] % (uint64_t)C.n);
@@ -184,6 +185,7 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
   res->m = m;
   res->dtype = pred->dtype;
   for (int j = 0; j < d; ++j) res->gdims[j] = mesh->global_dims[j];
+  res->tstride = mesh->time_id_stride;
   // ContourPredicate::extract_it stores its threshold as the scalar (predicate.hpp:150)
   if (pred->kind == FTK2H_PRED_CONTOUR) res->scalar_value = (float)pred->thresholds[0];
   if (!has_cells) {  // no hypercube: the reference visits nothing

@@ -152,6 +152,7 @@ struct ftk2h_result {
   int device = 0;
   int kind = 0, d = 0, m = 0, dtype = 0;
   uint64_t gdims[4] = {1, 1, 1, 1};
+  uint64_t tstride = 0;  // id step of the last axis when extruded (0: product of gdims)
   uint64_t n_nodes = 0;
   int ntypes = 0;
   uint64_t* keys = nullptr;
@@ -241,6 +242,13 @@ inline int check_mesh(const ftk2h_mesh_desc* mesh, int d, uint64_t* nvert, bool*
     n *= mesh->local_dims[j];
     hc = hc && mesh->local_dims[j] >= 2;
   }
+  if (mesh->time_id_stride) {
+    uint64_t sp = 1;
+    for (int j = 0; j + 1 < d; ++j) sp *= mesh->global_dims[j];
+    if (mesh->time_id_stride < sp) return fail(FTK2H_E_INVALID, "time_id_stride below the spatial vertex count");
+    if (mesh->global_dims[d - 1] > (1ull << 57) / mesh->time_id_stride)
+      return fail(FTK2H_E_INVALID, "vertex ids beyond 2^57");
+  }
   *nvert = n;
   *has_cells = hc;
   return FTK2H_OK;
@@ -259,6 +267,7 @@ inline void init_params(KParams& P, const ftk2h_mesh_desc* mesh, const ftk2h_pre
     P.gstride[j] = gs;
     gs *= (int64_t)mesh->global_dims[j];
   }
+  if (mesh->time_id_stride) P.gstride[d - 1] = (int64_t)mesh->time_id_stride;  // extruded layers
   P.nvert = (int64_t)nvert;
   P.small = nvert < (1ull << 31);
   P.Q = pred->sos_q;

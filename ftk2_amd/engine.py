@@ -53,6 +53,45 @@ class RegularSimplicialMesh:
         return d
 
 
+class ExtrudedSimplicialMesh:
+    """A RegularSimplicialMesh base stacked in n_layers + 1 time layers
+    (mesh.hpp:542-560,711-726): the layers are one more grid axis and
+    vertex id = spatial id + t * n_spatial_verts_, with n_spatial_verts_ =
+    grid_index_to_id(base global dims) (larger than the spatial vertex count).
+    Arrays are indexed [x, y, (z), t] at global coordinates, so the base must
+    have no offset."""
+
+    def __init__(self, base: RegularSimplicialMesh, n_layers: int):
+        if any(base.offset):
+            raise ValueError("ExtrudedSimplicialMesh: the base must have no offset")
+        self.base = base
+        self.n_layers = int(n_layers)
+        self.dims = list(base.dims) + [self.n_layers + 1]
+        self.offset = [0] * len(self.dims)
+        self.global_dims = list(base.global_dims) + [self.n_layers + 1]
+        r, mul = 0, 1
+        for g in base.global_dims:
+            r += g * mul
+            mul *= g
+        self.time_stride = r
+
+    def get_total_dimension(self):
+        return len(self.dims)
+
+    def get_spatial_dimension(self):
+        return len(self.base.dims)
+
+    def desc(self):
+        d = _lib.MeshDesc()
+        d.ndims = len(self.dims)
+        for j in range(4):
+            d.local_dims[j] = self.dims[j] if j < len(self.dims) else 1
+            d.offset[j] = 0
+            d.global_dims[j] = self.global_dims[j] if j < len(self.dims) else 1
+        d.time_id_stride = self.time_stride
+        return d
+
+
 @dataclass
 class CriticalPointPredicate:
     M: int = 3
@@ -202,14 +241,16 @@ class Result:
             scal.ctypes.data, track.ctypes.data))
         return verts, lam, bary, scal, track
 
-    def positions(self):
+    def positions(self, with_time=False):
         """(n, 3) float32 node positions as the reference's VTK writer computes them
-        (computed on the device)."""
+        (computed on the device); with_time: also the "Time" array (the layer
+        coordinate on an extruded mesh, else 0)."""
         n, _, _ = self.sizes()
         xyz = np.zeros((n, 3), np.float32)
+        tm = np.zeros(n, np.float32)
         if n:
-            _lib.check(_lib.lib().ftk2h_result_copy_positions(self.h, xyz.ctypes.data))
-        return xyz
+            _lib.check(_lib.lib().ftk2h_result_copy_positions(self.h, xyz.ctypes.data, tm.ctypes.data))
+        return (xyz, tm) if with_time else xyz
 
     def track_ids(self):
         n, _, _ = self.sizes()

@@ -7,7 +7,8 @@ the C++ twin is include/ftk2/utils/vtk.hpp here):
   Scalar   float32  FeatureElement::scalar
   TrackID  int32    (int)track_id
   Type     int32    FeatureElement::type (0 for every predicate on this path)
-  Time     float32  0 (a RegularSimplicialMesh has as many spatial as total dims)
+  Time     float32  the layer coordinate on an ExtrudedSimplicialMesh (more total
+                    than spatial dims), else 0 (a RegularSimplicialMesh)
   .vtu     target_dim 0: one VTK_VERTEX per node, else connectivity dims 0..3
   .vtp     verts / lines / polys from connectivity dims 0 / 1 / 2
 
@@ -28,24 +29,42 @@ VTK_TYPE = {0: 1, 1: 3, 2: 5, 3: 10}  # VERTEX, LINE, TRIANGLE, TETRA
 _NP = {"Float32": np.float32, "Float64": np.float64, "Int32": np.int32, "Int64": np.int64, "UInt8": np.uint8}
 
 
-def positions(vertices, bary, global_dims):
+def vertex_coordinates(ids, global_dims, time_stride=0):
+    """get_vertex_coordinates of vertex ids (..., d) float64: x fastest over the
+    global dims (mesh.hpp:374-378), or on an extruded mesh the layer t = id / N and
+    the base coordinates of id % N (mesh.hpp:711-717)."""
+    rest = np.asarray(ids, dtype=np.uint64).copy()
+    gd = [int(g) for g in global_dims]
+    out = np.zeros(rest.shape + (len(gd),), np.float64)
+    sd = len(gd)
+    if time_stride:
+        out[..., -1] = (rest // np.uint64(time_stride)).astype(np.float64)
+        rest %= np.uint64(time_stride)
+        sd -= 1
+    for j in range(sd):
+        out[..., j] = (rest % np.uint64(gd[j])).astype(np.float64)
+        rest //= np.uint64(gd[j])
+    return out
+
+
+def positions(vertices, bary, global_dims, time_stride=0, with_time=False):
     """Reference positions of nodes given their sorted vertex ids (n, m+1) and
-    float barycentric coordinates (n, m+1): (n, 3) float32."""
+    float barycentric coordinates (n, m+1): (n, 3) float32, and with_time the
+    "Time" array (the last coordinate on an extruded mesh, else 0)."""
     vertices = np.asarray(vertices, dtype=np.uint64)
     n, nv = vertices.shape
-    gd = [int(g) for g in global_dims]
-    pos = np.zeros((n, len(gd)), np.float64)
+    co = vertex_coordinates(vertices, global_dims, time_stride)
+    pos = np.zeros((n, co.shape[-1]), np.float64)
     for i in range(nv):
-        rest = vertices[:, i].copy()
         b = np.asarray(bary[:, i], dtype=np.float32).astype(np.float64)
-        for j, g in enumerate(gd):
-            c = (rest % np.uint64(g)).astype(np.float64)
-            rest //= np.uint64(g)
-            pos[:, j] = pos[:, j] + b * c
+        pos = pos + b[:, None] * co[:, i, :]
     out = np.zeros((n, 3), np.float32)
-    k = min(3, len(gd))
+    k = min(3, co.shape[-1])
     out[:, :k] = pos[:, :k].astype(np.float32)
-    return out
+    if not with_time:
+        return out
+    tm = pos[:, -1].astype(np.float32) if time_stride else np.zeros(n, np.float32)
+    return out, tm
 
 
 class _Writer:
@@ -84,30 +103,32 @@ def _mode(ascii_mode):
     return bool(ascii_mode) or os.environ.get("FTK_DEBUG") is not None
 
 
-def _point_section(lines, w, points, scalar, track, ntype):
-    n = len(points)
+def _point_section(lines, w, points, times, scalar, track, ntype):
     lines.append("      <PointData>")
     w.array(lines, "Float32", "Scalar", scalar)
     w.array(lines, "Int32", "TrackID", np.asarray(track).astype(np.int64).astype(np.int32))
     w.array(lines, "Int32", "Type", ntype)
-    w.array(lines, "Float32", "Time", np.zeros(n, np.float32))
+    w.array(lines, "Float32", "Time", times)
     lines.append("      </PointData>")
     lines.append("      <Points>")
     w.array(lines, "Float32", "Points", points, 3)
     lines.append("      </Points>")
 
 
-def _inputs(cx, global_dims, points):
-    if points is None:
-        points = positions(cx.vertices, cx.bary, global_dims)
+def _inputs(cx, global_dims, points, times, time_stride):
+    if points is None or times is None:
+        p, t = positions(cx.vertices, cx.bary, global_dims, time_stride, with_time=True)
+        points = p if points is None else points
+        times = t if times is None else times
     n = len(cx.vertices)
-    return points, np.asarray(cx.scalar, np.float32), np.asarray(cx.track_id), np.zeros(n, np.int32)
+    return points, times, np.asarray(cx.scalar, np.float32), np.asarray(cx.track_id), np.zeros(n, np.int32)
 
 
-def write_vtu(cx, global_dims, filename, target_dim=-1, ascii=False, points=None):
+def write_vtu(cx, global_dims, filename, target_dim=-1, ascii=False, points=None, times=None, time_stride=0):
     """write_complex_to_vtu (utils/vtk.hpp:118-160).  cx: ftk2_amd.engine.FeatureComplex;
-    points: positions computed elsewhere (e.g. Result.positions()), else computed here."""
-    points, scalar, track, ntype = _inputs(cx, global_dims, points)
+    points / times: positions and Time computed elsewhere (Result.positions()), else
+    computed here; time_stride: the mesh's time_id_stride (extruded meshes)."""
+    points, times, scalar, track, ntype = _inputs(cx, global_dims, points, times, time_stride)
     n = len(points)
     conn, offs, types = [], [], []
     if target_dim == 0:
@@ -130,7 +151,7 @@ def write_vtu(cx, global_dims, filename, target_dim=-1, ascii=False, points=None
     lines = ['<?xml version="1.0"?>',
              '<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian" header_type="UInt64">',
              "  <UnstructuredGrid>", f'    <Piece NumberOfPoints="{n}" NumberOfCells="{len(types)}">']
-    _point_section(lines, w, points, scalar, track, ntype)
+    _point_section(lines, w, points, times, scalar, track, ntype)
     lines.append("      <Cells>")
     w.array(lines, "Int64", "connectivity", conn)
     w.array(lines, "Int64", "offsets", offs)
@@ -141,9 +162,9 @@ def write_vtu(cx, global_dims, filename, target_dim=-1, ascii=False, points=None
         f.write("\n".join(lines) + "\n")
 
 
-def write_vtp(cx, global_dims, filename, target_dim=-1, ascii=False, points=None):
+def write_vtp(cx, global_dims, filename, target_dim=-1, ascii=False, points=None, times=None, time_stride=0):
     """write_complex_to_vtp (utils/vtk.hpp:78-113)."""
-    points, scalar, track, ntype = _inputs(cx, global_dims, points)
+    points, times, scalar, track, ntype = _inputs(cx, global_dims, points, times, time_stride)
     n = len(points)
     sect = {}
     for dim in (0, 1, 2):
@@ -158,7 +179,7 @@ def write_vtp(cx, global_dims, filename, target_dim=-1, ascii=False, points=None
              "  <PolyData>",
              f'    <Piece NumberOfPoints="{n}" NumberOfVerts="{len(sect[0][1])}" NumberOfLines="{len(sect[1][1])}" '
              f'NumberOfStrips="0" NumberOfPolys="{len(sect[2][1])}">']
-    _point_section(lines, w, points, scalar, track, ntype)
+    _point_section(lines, w, points, times, scalar, track, ntype)
     for dim, name in ((0, "Verts"), (1, "Lines"), (2, "Polys")):
         lines.append(f"      <{name}>")
         w.array(lines, "Int64", "connectivity", sect[dim][0])

@@ -82,7 +82,24 @@ class SimplicialEngine {
     (void)var_names;  // the predicate's own variable names select the arrays
     clear_results();
     auto reg = std::dynamic_pointer_cast<RegularSimplicialMesh>(mesh_);
-    if (!reg) throw std::runtime_error("SimplicialEngine: only RegularSimplicialMesh is supported on the GPU");
+    tstride_ = 0;
+    if (auto ext = std::dynamic_pointer_cast<ExtrudedSimplicialMesh>(mesh_)) {
+      // the layers become one more grid axis; only the vertex ids differ
+      auto base = std::dynamic_pointer_cast<RegularSimplicialMesh>(ext->get_base_mesh());
+      bool origin = base != nullptr;
+      if (base)
+        for (auto o : base->get_offset()) origin = origin && o == 0;
+      if (!origin)
+        throw std::runtime_error("SimplicialEngine: extruded meshes need a RegularSimplicialMesh base without offset");
+      auto dims = base->get_local_dims();
+      auto gdims = base->get_global_dims();
+      dims.push_back(ext->get_n_layers() + 1);
+      gdims.push_back(ext->get_n_layers() + 1);
+      reg = std::make_shared<RegularSimplicialMesh>(dims, std::vector<uint64_t>{}, gdims);
+      tstride_ = ext->get_n_spatial_verts();
+    }
+    if (!reg)
+      throw std::runtime_error("SimplicialEngine: only RegularSimplicialMesh (or extruded over one) is supported on the GPU");
     if constexpr (!detail::pred_traits<PredicateType>::supported) {
       throw std::runtime_error("SimplicialEngine: predicate not supported on the GPU");
     } else {
@@ -98,6 +115,7 @@ class SimplicialEngine {
         md.offset[j] = j < d ? off[j] : 0;
         md.global_dims[j] = j < d ? gd[j] : 1;
       }
+      md.time_id_stride = tstride_;
       ftk2h_predicate_desc pd{};
       pd.kind = detail::pred_traits<PredicateType>::kind;
       pd.codimension = m;
@@ -250,7 +268,7 @@ class SimplicialEngine {
   const ftk2h_result* result() const { return result_; }
 
  private:
-  static ftk2h_field_view legacy_view(const ftk::ndarray<T>& arr, const RegularSimplicialMesh* reg) {
+  ftk2h_field_view legacy_view(const ftk::ndarray<T>& arr, const RegularSimplicialMesh* reg) const {
     // get_value (engine.hpp:999-1009): 1-D arrays by vertex id, else f(local coords)
     ftk2h_field_view v{};
     v.on_device = 0;
@@ -260,7 +278,7 @@ class SimplicialEngine {
       const auto& off = reg->get_offset();
       int64_t s = 1, base = 0;
       for (int j = 0; j < d; ++j) {
-        v.strides[j] = s;
+        v.strides[j] = tstride_ && j == d - 1 ? (int64_t)tstride_ : s;  // extruded ids
         base += (int64_t)off[j] * s;
         s *= (int64_t)gd[j];
       }
@@ -344,6 +362,7 @@ class SimplicialEngine {
   PredicateType predicate_;
   int device_ = 0;
   ftk2h_result* result_ = nullptr;
+  uint64_t tstride_ = 0;  // id step of the layer axis when mesh_ is extruded
 };
 
 }  // namespace ftk2

@@ -65,6 +65,11 @@ typedef struct {
   uint64_t local_dims[4];  /* vertices per axis of this mesh (slab) */
   uint64_t offset[4];      /* global coordinate of local (0,...,0) */
   uint64_t global_dims[4]; /* extents used for vertex ids */
+  uint64_t time_id_stride; /* 0: vertex id = x + gx*(y + gy*(z + gz*t)) (RegularSimplicialMesh).
+                              Nonzero: the last axis is a layer of an ExtrudedSimplicialMesh
+                              over a regular base, id = spatial id + t * time_id_stride with
+                              time_id_stride = n_spatial_verts_ (mesh.hpp:544-560); must be at
+                              least the spatial vertex count.  ftk2h_extract_regular only. */
 } ftk2h_mesh_desc;
 
 typedef struct {
@@ -164,10 +169,12 @@ int ftk2h_result_add_attributes(ftk2h_ctx* ctx, ftk2h_result* r, const ftk2h_mes
 int ftk2h_result_copy_attributes(const ftk2h_result* r, float* out);
 
 /* Feature positions as the reference's VTK writer computes them
- * (utils/vtk.hpp:26-49): xyz[3*i + j] = (float) sum_k (double)bary_k * coord_j of
+ * (utils/vtk.hpp:26-59): xyz[3*i + j] = (float) sum_k (double)bary_k * coord_j of
  * the node's vertices (global grid indices), first three coordinates, 0 beyond
- * the mesh dimension.  Host buffer of 3 * n_nodes floats. */
-int ftk2h_result_copy_positions(const ftk2h_result* r, float* xyz);
+ * the mesh dimension; time[i] (may be NULL) = the last coordinate when the mesh is
+ * extruded (time_id_stride != 0), else 0.  Host buffers of 3 * n_nodes and
+ * n_nodes floats. */
+int ftk2h_result_copy_positions(const ftk2h_result* r, float* xyz, float* time);
 
 /* ---- Streaming (SimplicialEngine::execute_stream, engine.hpp:214-270) ----------
  * Timesteps of a spatial grid arrive one at a time (ftk2h_stream_append, views

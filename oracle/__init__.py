@@ -37,7 +37,7 @@ def lib():
         P, U64P, I64P, DP = C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)
         L.oracle_run.restype = P
         L.oracle_run.argtypes = [C.c_int, U64P, U64P, U64P, C.c_int, C.c_int, C.c_int, C.c_double, DP,
-                                 C.c_int, C.POINTER(C.c_void_p), I64P, C.c_void_p, I64P, C.c_int]
+                                 C.c_int, C.POINTER(C.c_void_p), I64P, C.c_void_p, I64P, C.c_int, C.c_uint64]
         L.oracle_run_stream.restype = P
         L.oracle_run_stream.argtypes = [C.c_int, U64P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, DP,
                                         C.c_int, C.POINTER(C.c_void_p), C.c_int]
@@ -105,11 +105,13 @@ def _collect(h):
 
 
 def run(comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e6, scalar=None,
-        nthreads=0):
+        nthreads=0, time_stride=0):
     """Run the CPU engine on a regular mesh.
 
     comps: list of numpy arrays (float32 or float64), indexed [x, y, (z), t] on local
     coordinates, any strides (the reference's legacy separate-array mode).
+    time_stride: nonzero for an ExtrudedSimplicialMesh over a regular base (vertex
+    id = spatial id + t * time_stride, see extruded_time_stride).
     """
     comps = list(comps)
     dtype = 0 if comps[0].dtype == np.float32 else 1
@@ -136,8 +138,18 @@ def run(comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e
             sst[j] = scalar.strides[j] // item
     h = lib().oracle_run(d, dims_p, off_p, g_p, kind, m, dtype, Q, thr_p, len(comps), ptrs,
                          st.ctypes.data_as(C.POINTER(C.c_int64)), sptr,
-                         sst.ctypes.data_as(C.POINTER(C.c_int64)), nthreads)
+                         sst.ctypes.data_as(C.POINTER(C.c_int64)), nthreads, C.c_uint64(time_stride))
     return _collect(h)
+
+
+def extruded_time_stride(spatial_global_dims):
+    """ExtrudedSimplicialMesh::n_spatial_verts_ for a regular base
+    (mesh.hpp:547-552): grid_index_to_id(global dims) = sum_j g_j * prod_{i<j} g_i."""
+    r, mul = 0, 1
+    for g in spatial_global_dims:
+        r += int(g) * mul
+        mul *= int(g)
+    return r
 
 
 def run_stream(steps, kind, m, thr=(0.0, 0.0), Q=1e6, nthreads=0):

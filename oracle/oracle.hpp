@@ -99,17 +99,27 @@ inline Chains make_chains(int d, int m) {
 struct Mesh {
   int d = 0;
   uint64_t dims[4] = {1, 1, 1, 1}, off[4] = {0, 0, 0, 0}, gdims[4] = {1, 1, 1, 1};
+  // Nonzero: ExtrudedSimplicialMesh over a regular base (mesh.hpp:542-560,711-717):
+  // the last axis is the layer t and id = spatial id + t * tstride, where tstride
+  // is the base's n_spatial_verts_ = grid_index_to_id(global dims of the base).
+  uint64_t tstride = 0;
   uint64_t id(const uint64_t g[4]) const {
     uint64_t r = 0, mul = 1;
     for (int j = 0; j < d; ++j) {
-      r += g[j] * mul;
+      r += g[j] * (tstride && j == d - 1 ? tstride : mul);
       mul *= gdims[j];
     }
     return r;
   }
   void coords(uint64_t id_, uint64_t g[4]) const {  // mesh.hpp:420-424
     for (int j = 0; j < 4; ++j) g[j] = 0;
-    for (int j = 0; j < d; ++j) {
+    int sd = d;
+    if (tstride) {  // t = id / n_spatial_verts_, spatial = id % n_spatial_verts_
+      g[d - 1] = id_ / tstride;
+      id_ %= tstride;
+      sd = d - 1;
+    }
+    for (int j = 0; j < sd; ++j) {
       g[j] = id_ % gdims[j];
       id_ /= gdims[j];
     }

@@ -54,10 +54,12 @@ const char* oracle_error() { return g_err.c_str(); }
 
 void* oracle_run(int d, const uint64_t* dims, const uint64_t* off, const uint64_t* gdims, int kind, int m,
                  int dtype, double Q, const double* thr, int ncomp, const void* const* ptrs,
-                 const int64_t* strides, const void* sptr, const int64_t* sstrides, int nthreads) {
+                 const int64_t* strides, const void* sptr, const int64_t* sstrides, int nthreads,
+                 uint64_t tstride) {
   try {
     Mesh mesh;
     mesh.d = d;
+    mesh.tstride = tstride;
     for (int j = 0; j < d; ++j) {
       mesh.dims[j] = dims[j];
       mesh.off[j] = off ? off[j] : 0;

@@ -216,7 +216,51 @@ static void attributes_3d() {
   std::printf("attributes_3d: %zu vertices\n", cx.vertices.size());
 }
 
+// An ExtrudedSimplicialMesh over a regular 2-D base (what the reference's
+// FeatureTracker builds, feature_tracker.cpp:454) gives the regular 3-D mesh's
+// features with extruded vertex ids (id = spatial + t * n_spatial_verts_).
+static void extruded_2d() {
+  const int W = 24, H = 20, Tn = 6;
+  ftk::ndarray<double> u({(size_t)W, (size_t)H, (size_t)Tn}), v({(size_t)W, (size_t)H, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        u.f(x, y, t) = std::sin(0.37 * x + 0.2 * t) + 0.3 * std::cos(0.21 * y);
+        v.f(x, y, t) = std::cos(0.29 * y - 0.15 * t) + 0.3 * std::sin(0.17 * x);
+      }
+  CriticalPointPredicate<2, double> pred;
+  pred.var_names[0] = "u";
+  pred.var_names[1] = "v";
+  auto base = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{(uint64_t)W, (uint64_t)H});
+  auto ext = std::make_shared<ExtrudedSimplicialMesh>(base, Tn - 1);
+  CHECK_EQ(ext->get_n_spatial_verts(), (uint64_t)(W + W * H));  // grid_index_to_id({W, H})
+  SimplicialEngine<double, CriticalPointPredicate<2, double>> e1(ext, pred);
+  e1.execute({{"u", u}, {"v", v}});
+  auto cx = e1.get_complex();
+  auto reg = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{(uint64_t)W, (uint64_t)H, (uint64_t)Tn});
+  SimplicialEngine<double, CriticalPointPredicate<2, double>> e2(reg, pred);
+  e2.execute({{"u", u}, {"v", v}});
+  auto rx = e2.get_complex();
+  CHECK_EQ(cx.vertices.size(), rx.vertices.size());
+  CHECK_EQ(cx.vertices.size() > 0, true);
+  const uint64_t N = ext->get_n_spatial_verts(), S = (uint64_t)W * H;
+  for (size_t i = 0; i < cx.vertices.size() && i < rx.vertices.size(); ++i) {
+    for (int k = 0; k <= 2; ++k) {
+      const uint64_t id = cx.vertices[i].simplex.vertices[k];
+      CHECK_EQ((id % N) + (id / N) * S, rx.vertices[i].simplex.vertices[k]);
+    }
+    CHECK_EQ(cx.vertices[i].track_id, rx.vertices[i].track_id);
+    CHECK_EQ(cx.vertices[i].barycentric_coords[0][0], rx.vertices[i].barycentric_coords[0][0]);
+  }
+  CHECK_EQ(cx.connectivity.size(), rx.connectivity.size());
+  for (size_t c = 0; c < cx.connectivity.size() && c < rx.connectivity.size(); ++c)
+    CHECK_EQ(cx.connectivity[c].indices == rx.connectivity[c].indices, true);
+  check_complex(cx);
+  std::printf("extruded_2d: %zu vertices\n", cx.vertices.size());
+}
+
 int main() {
+  extruded_2d();
   attributes_3d();
   levelset_3d();
   fiber_3d();

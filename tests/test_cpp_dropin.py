@@ -40,3 +40,4 @@ def test_cpp_examples_on_gpu():
     assert "66418" in p.stdout and "6078" in p.stdout
     assert "streaming:" in p.stdout  # execute_stream == execute through ftk::stream<>
     assert "attributes_3d:" in p.stdout
+    assert "extruded_2d:" in p.stdout  # ExtrudedSimplicialMesh over a regular base
