@@ -60,6 +60,74 @@ def cpu_baseline(comps, n_sample):
                       f"({n_simp} tets, {len(r['verts'])} nodes, {dt:.2f} s)"}
 
 
+SCALAR_WORKLOADS = {
+    # name: (predicate kind, m, simplex types per hypercube, side, timesteps, field seeds, threshold(s))
+    "contour": (1, 1, 15, 384, 4, (0,), (0.0, 0.0)),
+    "fiber": (2, 2, 50, 256, 4, (1, 2), (0.0, 0.0)),
+}
+
+
+def scalar_workload(args, name):
+    """The contour (384^3 x 4 FP32, isovalue 0) and fiber (two 256^3 x 4 FP32 fields)
+    workloads: fields = sum of 64 moving Gaussians minus the mean, inputs resident
+    in HBM; same step and JSON schema as the main line."""
+    import torch
+    import oracle
+    from ftk2_amd import engine, synth
+    kind, m, ntypes, n, nt, seeds, thr = SCALAR_WORKLOADS[name]
+    log(f"[bench] generating {len(seeds)} x {n}^3 x {nt} FP32 field(s)")
+    comps = [synth.scalar_config(n=n, nt=nt, seed=s) for s in seeds]
+    dev = [torch.from_numpy(np.ascontiguousarray(c.transpose())).cuda().permute(3, 2, 1, 0) for c in comps]
+    torch.cuda.synchronize()
+    mesh = engine.RegularSimplicialMesh([n, n, n, nt])
+    for i in range(args.warmup):
+        r = engine.extract(mesh, kind, m, dev, thresholds=thr)
+        log(f"[bench] warmup {i}: phases(ms)={['%.3f' % x for x in r.phase_ms()]}")
+        del r
+    torch.cuda.synchronize()
+    phases = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = engine.extract(mesh, kind, m, dev, thresholds=thr)
+        phases.append(r.phase_ms())
+        dv = r.device_view()
+        n_nodes, n_cells = dv.n_nodes, (dv.conn_cells[0] if dv.n_conn else 0)
+        del r
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ph = np.mean(np.array(phases), axis=0)
+    log(f"[bench] {name}: ms/step={dt * 1e3:.3f} phases(ms)={['%.3f' % x for x in ph]} nodes={n_nodes} "
+        f"cells={n_cells}")
+    n_simp = (n - 1) ** 3 * (nt - 1) * ntypes
+    in_bytes = len(comps) * n ** 3 * nt * 4
+    achieved = in_bytes / (float(ph[0]) * 1e-3) / 1e9
+    out = {
+        "metric": f"simplices/s, {name} {n}^3x{nt} FP32",
+        "value": n_simp / dt, "unit": "simplices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{name}_{n}^3x{nt}_fp32", "fields": len(comps), "gaussians": 64,
+                   "thresholds": list(thr[:len(comps)]), "nodes": int(n_nodes), "cells": int(n_cells)},
+        "roofline": {"bound": "hbm", "kernel": "k_classify+k_stage2", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": int(in_bytes)},
+        "pipeline": {"phase_ms": [float(x) for x in ph]},
+    }
+    if not args.no_cpu:
+        s = args.cpu_sample
+        sub = [np.asfortranarray(c[:s, :s, :s, :]) for c in comps]
+        threads = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1, 16)
+        log(f"[bench] cpu baseline on a {s}^3 x {nt} sample")
+        t0 = time.perf_counter()
+        ref = oracle.run(sub, {1: oracle.PRED_CONTOUR, 2: oracle.PRED_FIBER}[kind], m, thr=thr, nthreads=threads)
+        cdt = time.perf_counter() - t0
+        ns = (s - 1) ** 3 * (nt - 1) * ntypes
+        out["cpu_baseline"] = {"value": ns / cdt, "unit": "simplices/s", "cores": threads, "kind": "port",
+                               "sample": f"{s}^3x{nt} corner block of the same field(s) ({ns} simplices, "
+                                         f"{len(ref['verts'])} nodes, {cdt:.2f} s)"}
+    return out
+
+
 def batch_leg(seed, njobs, reps, log_every=256):
     """1024 independent CriticalPointPredicate<2,double> jobs (spatial sides log-uniform
     in [32, 1024], timesteps uniform in [2, 16], 4-64 moving Gaussians): per-job plain
@@ -239,6 +307,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (PMC calibration)")
+    ap.add_argument("--workload", choices=["cp3d", "contour", "fiber"], default="cp3d",
+                    help="cp3d (the benchmark line), or the contour / fiber workloads on their own")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -248,6 +318,9 @@ def main():
     import torch
     from ftk2_amd import engine, synth
 
+    if args.workload != "cp3d":
+        print(json.dumps(scalar_workload(args, args.workload)), flush=True)
+        return
     n, nt = args.n, args.nt
     log(f"[bench] generating {n}^3 x {nt} FP64 field (seed {args.seed})")
     t0 = time.perf_counter()

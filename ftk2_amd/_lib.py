@@ -46,7 +46,8 @@ class DeviceView(C.Structure):
     _fields_ = [("n_nodes", C.c_uint64), ("node_dim", C.c_int), ("keys", C.c_void_p),
                 ("lambda_", C.c_void_p), ("track_id", C.c_void_p), ("scalar", C.c_void_p),
                 ("n_conn", C.c_int), ("conn_dim", C.c_int * 3), ("conn_cells", C.c_uint64 * 3),
-                ("conn", C.c_void_p * 3), ("scalar_value", C.c_float)]
+                ("conn", C.c_void_p * 3), ("scalar_value", C.c_float), ("n_types", C.c_int),
+                ("global_dims", C.c_uint64 * 4), ("time_id_stride", C.c_uint64)]
 
 
 class AttributeDesc(C.Structure):

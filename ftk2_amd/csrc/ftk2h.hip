@@ -348,6 +348,9 @@ int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out) {
   out->track_id = r->track;
   out->scalar = r->scalar;
   out->scalar_value = r->scalar_value;
+  out->n_types = r->ntypes;
+  for (int j = 0; j < 4; ++j) out->global_dims[j] = r->gdims[j];
+  out->time_id_stride = r->tstride;
   out->n_conn = r->n_conn;
   for (int i = 0; i < 3; ++i) {
     out->conn_dim[i] = r->conn_dim[i];

@@ -112,6 +112,9 @@ typedef struct {
   const uint32_t* conn[3];   /* conn_cells * (conn_dim+1) canonical indices, sorted */
   float scalar_value;        /* FeatureElement::scalar of every node when scalar is NULL
                                 (the contour threshold, else 0) */
+  int n_types;               /* key = v_min * n_types + type */
+  uint64_t global_dims[4];   /* vertex id extents ... */
+  uint64_t time_id_stride;   /* ... and the last axis' id step when extruded (else 0) */
 } ftk2h_device_view;
 
 int ftk2h_ctx_create(int device, ftk2h_ctx** out);

@@ -35,6 +35,9 @@ def compare(name, comps, kind, m, thr=(0.0, 0.0)):
     assert sorted(cx.connectivity) == sorted(ref["conn"])
     for d in ref["conn"]:
         assert np.array_equal(cx.connectivity[d], ref["conn"][d])
+    dv = res.device_view()  # what a consumer needs to decode keys on the device
+    assert dv.time_id_stride == mesh.time_stride and dv.n_types == len(oracle.chains(len(mesh.dims), m))
+    assert list(dv.global_dims)[:len(mesh.dims)] == mesh.global_dims
     p, t = res.positions(with_time=True)
     ep, et = vtk.positions(cx.vertices, cx.bary, mesh.global_dims, mesh.time_stride, with_time=True)
     assert np.array_equal(p, ep) and np.array_equal(t, et)
