@@ -1085,14 +1085,15 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
     slot[threadIdx.x] = acc;
     prom[threadIdx.x] = 0;
     if constexpr (M >= 2) {
-      run_items(il, w & deg, [&](int owner, int t) {
-        if (item_inside<KIND, D, M, T>(P, tab, refs[owner], t)) {
-          if constexpr (KIND == KIND_FIBER)
-            atomicOr(&slot[owner], 1ull << t);
-          else
-            atomicOr(&prom[owner], 1ull << t);
-        }
-      });
+      if (__syncthreads_or((w & deg) != 0))  // phase A only when the tile has such items
+        run_items(il, w & deg, [&](int owner, int t) {
+          if (item_inside<KIND, D, M, T>(P, tab, refs[owner], t)) {
+            if constexpr (KIND == KIND_FIBER)
+              atomicOr(&slot[owner], 1ull << t);
+            else
+              atomicOr(&prom[owner], 1ull << t);
+          }
+        });
     }
     if constexpr (KIND != KIND_FIBER) {
       uint64_t ws = (w & ~deg) | prom[threadIdx.x];
@@ -1112,9 +1113,10 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
         ws = prom[threadIdx.x];
       }
       // Phase B2: the solver gate itself
-      run_items(il, ws, [&](int owner, int t) {
-        if (item_solves<KIND, D, M, T>(P, tab, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
-      });
+      if (__syncthreads_or(ws != 0))
+        run_items(il, ws, [&](int owner, int t) {
+          if (item_solves<KIND, D, M, T>(P, tab, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
+        });
     }
     acc = slot[threadIdx.x];
     if (v < P.nvert) gptr(P.mask)[v] = acc;
