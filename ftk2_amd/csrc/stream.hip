@@ -8,9 +8,12 @@
 //     time t) are also nodes of the next slab, so that slab stays open until the
 //     next one arrives and its edges can be re-indexed by key;
 //   * the committed nodes/edges and a union-find over them (the open frontier of
-//     the track components).
+//     the track components).  Once the committed payload on the device exceeds the
+//     retirement threshold, finished nodes, lambda and cells move to host memory and
+//     the union-find keeps only the frontier and its open roots (frontier.hpp).
 // The result of finish() equals the one-shot extraction over all timesteps (the
 // reference's contract, test_examples.cpp:259-297), which the tests check.
+#include "frontier.hpp"
 #include "stream.hpp"
 
 using namespace ftk2h;
@@ -61,12 +64,13 @@ __global__ void k_lower_bound(const uint64_t* keys, uint64_t n, uint64_t bound, 
 
 // Re-index the cells of a slab result to committed indices: lower-plane nodes are
 // committed at base + i; upper-plane nodes are found by key among the next slab's
-// lower-plane nodes (committed at next_base + position).  Each cell is then united.
+// lower-plane nodes (committed at next_base + position).  Each cell is then united
+// in the union-find, whose slot of a live node is its committed index - shift.
 template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, uint64_t nc, const uint64_t* keys,
                                                        uint32_t nlow, uint32_t base, const uint64_t* next_keys,
                                                        uint32_t next_nlow, uint32_t next_base, uint32_t* out,
-                                                       uint32_t* parent, uint32_t* bad) {
+                                                       uint32_t* parent, uint32_t shift, uint32_t* bad) {
   const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nc) return;
   uint32_t c[NV];
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, ui
   for (int q = 0; q < NV; ++q) out[i * NV + q] = c[q];
   if (!ok) return;  // reported by ftk2h_stream_finish; indices may lie beyond the parents
 #pragma unroll
-  for (int q = 1; q < NV; ++q) uf_unite(parent, c[0], c[q]);
+  for (int q = 1; q < NV; ++q) uf_unite(parent, c[0] - shift, c[q] - shift);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_iota(uint32_t* p, uint32_t base, uint64_t n) {
@@ -126,7 +130,15 @@ struct ftk2h_stream {
   uint32_t open_nlow = 0;
   uint32_t committed = 0;                         // committed nodes
   uint64_t committed_cells = 0;
-  DevVec keys, lambda, parent, cells;
+  // device: keys/lambda of committed nodes [fr.F, committed), cells from
+  // retired_cells on, parents over the union-find slots (frontier.hpp)
+  DevVec keys, lambda, parent, cells, roots;
+  ftk2h_frontier::Frontier fr;
+  uint64_t retired_cells = 0;
+  uint64_t retire_bytes = (uint64_t)1 << 30;  // device payload that triggers retirement
+  std::vector<uint64_t> h_keys;               // retired payload, in committed order
+  std::vector<char> h_lambda;
+  std::vector<uint32_t> h_cells;
   uint32_t* scratch = nullptr;  // small device words (lower bound, bad-key count)
   float ms_total = 0.f;
 };
@@ -148,45 +160,94 @@ int commit_open(ftk2h_stream* S, const ftk2h_result* next, uint32_t next_nlow) {
   const uint32_t nk = next ? S->open_nlow : (uint32_t)R->n_nodes;
   const size_t lsz = (size_t)(R->m + 1) * S->esz;
   int rc;
-  const uint64_t n_after = (uint64_t)S->committed + nk + (next ? next_nlow : 0);
-  if ((rc = S->keys.reserve(n_after * 8, S->st))) return rc;
-  if ((rc = S->lambda.reserve(n_after * lsz, S->st))) return rc;
-  if ((rc = S->parent.reserve(n_after * 4, S->st))) return rc;
+  const uint64_t dev_nodes = (uint64_t)S->committed - S->fr.F;  // committed nodes still on the device
+  const uint32_t shift = (uint32_t)S->fr.shift();
+  if ((rc = S->keys.reserve((dev_nodes + nk) * 8, S->st))) return rc;
+  if ((rc = S->lambda.reserve((dev_nodes + nk) * lsz, S->st))) return rc;
+  // parents of this slab's nodes and of the next slab's lower plane (cells point there)
+  const uint64_t fresh = nk + (next ? next_nlow : 0);
+  const uint64_t slots = (uint64_t)S->committed - shift + fresh;
+  if ((rc = S->parent.reserve(slots * 4, S->st))) return rc;
   char* kb = static_cast<char*>(S->keys.p);
   char* lb = static_cast<char*>(S->lambda.p);
   if (nk) {
-    HIP_TRY(hipMemcpyAsync(kb + (size_t)S->committed * 8, R->keys, (size_t)nk * 8, hipMemcpyDeviceToDevice, S->st));
-    HIP_TRY(hipMemcpyAsync(lb + (size_t)S->committed * lsz, R->lambda, (size_t)nk * lsz, hipMemcpyDeviceToDevice,
-                           S->st));
+    HIP_TRY(hipMemcpyAsync(kb + dev_nodes * 8, R->keys, (size_t)nk * 8, hipMemcpyDeviceToDevice, S->st));
+    HIP_TRY(hipMemcpyAsync(lb + dev_nodes * lsz, R->lambda, (size_t)nk * lsz, hipMemcpyDeviceToDevice, S->st));
   }
-  // parents of this slab's nodes and of the next slab's lower plane (cells point there)
-  const uint64_t fresh = nk + (next ? next_nlow : 0);
   uint32_t* par = static_cast<uint32_t*>(S->parent.p);
-  const uint64_t already = S->parent.bytes / 4;  // parents initialised so far
-  if ((uint64_t)S->committed + fresh > already) {
-    const uint64_t from = already, cnt = (uint64_t)S->committed + fresh - already;
-    k_iota<<<(unsigned)((cnt + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(par + from, (uint32_t)from, cnt);
-    S->parent.bytes = ((uint64_t)S->committed + fresh) * 4;
+  const uint64_t already = S->parent.bytes / 4;  // slots initialised so far
+  if (slots > already) {
+    const uint64_t cnt = slots - already;
+    k_iota<<<(unsigned)((cnt + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(par + already, (uint32_t)already, cnt);
+    S->parent.bytes = slots * 4;
   }
   if (S->nv && R->n_conn) {
     const uint64_t nc = R->conn_cells[0];
-    if ((rc = S->cells.reserve((S->committed_cells + nc) * S->nv * 4, S->st))) return rc;
-    uint32_t* dst = static_cast<uint32_t*>(S->cells.p) + S->committed_cells * S->nv;
+    const uint64_t dev_cells = S->committed_cells - S->retired_cells;
+    if ((rc = S->cells.reserve((dev_cells + nc) * S->nv * 4, S->st))) return rc;
+    uint32_t* dst = static_cast<uint32_t*>(S->cells.p) + dev_cells * S->nv;
     const unsigned g = (unsigned)((nc + BLOCK - 1) / BLOCK);
     const uint64_t* nkeys = next ? next->keys : nullptr;
     const uint32_t nb = S->committed + nk;
     switch (S->nv) {
-      case 2: k_remap_cells<2><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
-      case 3: k_remap_cells<3><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
-      case 4: k_remap_cells<4><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, S->scratch + 1); break;
+      case 2: k_remap_cells<2><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, shift, S->scratch + 1); break;
+      case 3: k_remap_cells<3><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, shift, S->scratch + 1); break;
+      case 4: k_remap_cells<4><<<g, BLOCK, 0, S->st>>>(R->conn[0], nc, R->keys, nk, S->committed, nkeys, next_nlow, nb, dst, par, shift, S->scratch + 1); break;
     }
     HIP_TRY(hipGetLastError());
     S->committed_cells += nc;
-    S->cells.bytes = S->committed_cells * S->nv * 4;
+    S->cells.bytes = (S->committed_cells - S->retired_cells) * S->nv * 4;
   }
   S->committed += nk;
-  S->keys.bytes = (size_t)S->committed * 8;
-  S->lambda.bytes = (size_t)S->committed * lsz;
+  S->keys.bytes = ((size_t)S->committed - S->fr.F) * 8;
+  S->lambda.bytes = ((size_t)S->committed - S->fr.F) * lsz;
+  return FTK2H_OK;
+}
+
+// Root slot of every union-find slot, copied to host.
+int slot_roots(ftk2h_stream* S, std::vector<uint32_t>& out) {
+  const uint64_t P = S->parent.bytes / 4;
+  out.assign(P, 0);
+  if (!P) return FTK2H_OK;
+  int rc;
+  if ((rc = S->roots.reserve(P * 4, S->st))) return rc;
+  k_compress<<<(unsigned)((P + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(static_cast<uint32_t*>(S->parent.p),
+                                                                       static_cast<uint32_t*>(S->roots.p), P);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out.data(), S->roots.p, P * 4, hipMemcpyDeviceToHost, S->st));
+  HIP_TRY(hipStreamSynchronize(S->st));
+  return FTK2H_OK;
+}
+
+// Move every committed node, lambda and cell on the device to host memory; the
+// union-find keeps the next slab's lower plane and the open roots (frontier.hpp).
+// Runs right after a commit against a next slab, so no later cell references a
+// retired node.
+int retire(ftk2h_stream* S) {
+  const size_t lsz = (size_t)(S->pred.codimension + 1) * S->esz;
+  const uint64_t nret = (uint64_t)S->committed - S->fr.F;
+  const uint64_t ncell = (S->committed_cells - S->retired_cells) * S->nv;
+  std::vector<uint32_t> roots;
+  int rc;
+  if ((rc = slot_roots(S, roots))) return rc;
+  const size_t k0 = S->h_keys.size(), l0 = S->h_lambda.size(), c0 = S->h_cells.size();
+  S->h_keys.resize(k0 + nret);
+  S->h_lambda.resize(l0 + nret * lsz);
+  S->h_cells.resize(c0 + ncell);
+  if (nret) {
+    HIP_TRY(hipMemcpyAsync(S->h_keys.data() + k0, S->keys.p, nret * 8, hipMemcpyDeviceToHost, S->st));
+    HIP_TRY(hipMemcpyAsync(S->h_lambda.data() + l0, S->lambda.p, nret * lsz, hipMemcpyDeviceToHost, S->st));
+  }
+  if (ncell) HIP_TRY(hipMemcpyAsync(S->h_cells.data() + c0, S->cells.p, ncell * 4, hipMemcpyDeviceToHost, S->st));
+  HIP_TRY(hipStreamSynchronize(S->st));
+  const std::vector<uint32_t> init = S->fr.retire(roots, roots.size() - S->fr.K(), S->committed);
+  if (!init.empty()) {
+    HIP_TRY(hipMemcpyAsync(S->parent.p, init.data(), init.size() * 4, hipMemcpyHostToDevice, S->st));
+    HIP_TRY(hipStreamSynchronize(S->st));
+  }
+  S->parent.bytes = init.size() * 4;
+  S->keys.bytes = S->lambda.bytes = S->cells.bytes = 0;
+  S->retired_cells = S->committed_cells;
   return FTK2H_OK;
 }
 
@@ -290,6 +351,7 @@ int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow) {
     rc = commit_open(S, R, nlow);
     ftk2h_result_free(S->open);
     S->open = nullptr;
+    if (!rc && S->keys.bytes + S->lambda.bytes + S->cells.bytes > S->retire_bytes) rc = retire(S);
     if (rc) {
       ftk2h_result_free(R);
       return rc;
@@ -305,7 +367,7 @@ uint64_t stream_timesteps(const ftk2h_stream* S) { return S->t; }
 
 uint64_t stream_device_bytes(const ftk2h_stream* S) {
   uint64_t b = 2 * S->nsp * S->esz * (uint64_t)S->ncomp;
-  b += S->keys.cap + S->lambda.cap + S->parent.cap + S->cells.cap;
+  b += S->keys.cap + S->lambda.cap + S->parent.cap + S->cells.cap + S->roots.cap;
   return b;
 }
 
@@ -404,23 +466,46 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
   if (S->pred.kind == FTK2H_PRED_CONTOUR) res->scalar_value = (float)S->pred.thresholds[0];
   const uint64_t n = S->committed;
   res->n_nodes = n;
+  // retired payload first (host), then what is still on the device
+  const uint64_t F = S->fr.F;
+  std::vector<uint32_t> tracks;
+  if (S->fr.active()) {
+    std::vector<uint32_t> roots;
+    if ((rc = slot_roots(S, roots))) return rc;
+    tracks = S->fr.tracks(roots, roots.size() - S->fr.K());
+    if (tracks.size() != n) return fail(FTK2H_E_HIP, "stream: retired union-find does not cover the nodes");
+  }
   if (n) {
     const size_t lsz = (size_t)(res->m + 1) * S->esz;
     HIP_TRY(hipMalloc(&res->keys, n * 8));
     HIP_TRY(hipMalloc(&res->lambda, n * lsz));
     HIP_TRY(hipMalloc(&res->track, n * 4));
-    HIP_TRY(hipMemcpyAsync(res->keys, S->keys.p, n * 8, hipMemcpyDeviceToDevice, S->st));
-    HIP_TRY(hipMemcpyAsync(res->lambda, S->lambda.p, n * lsz, hipMemcpyDeviceToDevice, S->st));
-    k_compress<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(static_cast<uint32_t*>(S->parent.p),
-                                                                         res->track, n);
-    HIP_TRY(hipGetLastError());
+    if (F) {
+      HIP_TRY(hipMemcpyAsync(res->keys, S->h_keys.data(), F * 8, hipMemcpyHostToDevice, S->st));
+      HIP_TRY(hipMemcpyAsync(res->lambda, S->h_lambda.data(), F * lsz, hipMemcpyHostToDevice, S->st));
+    }
+    if (n > F) {
+      HIP_TRY(hipMemcpyAsync(res->keys + F, S->keys.p, (n - F) * 8, hipMemcpyDeviceToDevice, S->st));
+      HIP_TRY(hipMemcpyAsync(static_cast<char*>(res->lambda) + F * lsz, S->lambda.p, (n - F) * lsz,
+                             hipMemcpyDeviceToDevice, S->st));
+    }
+    if (S->fr.active()) {
+      HIP_TRY(hipMemcpyAsync(res->track, tracks.data(), n * 4, hipMemcpyHostToDevice, S->st));
+      HIP_TRY(hipStreamSynchronize(S->st));  // `tracks` is local
+    } else {
+      k_compress<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(static_cast<uint32_t*>(S->parent.p),
+                                                                           res->track, n);
+      HIP_TRY(hipGetLastError());
+    }
   }
   if (S->nv && S->committed_cells) {
     res->n_conn = 1;
     res->conn_dim[0] = S->nv - 1;
     res->conn_cells[0] = S->committed_cells;
+    const uint64_t hc = S->h_cells.size(), dc = S->committed_cells * S->nv - hc;
     HIP_TRY(hipMalloc(&res->conn[0], S->committed_cells * S->nv * 4));
-    HIP_TRY(hipMemcpyAsync(res->conn[0], S->cells.p, S->committed_cells * S->nv * 4, hipMemcpyDeviceToDevice, S->st));
+    if (hc) HIP_TRY(hipMemcpyAsync(res->conn[0], S->h_cells.data(), hc * 4, hipMemcpyHostToDevice, S->st));
+    if (dc) HIP_TRY(hipMemcpyAsync(res->conn[0] + hc, S->cells.p, dc * 4, hipMemcpyDeviceToDevice, S->st));
     if (S->nv == 3 && S->pred.kind != FTK2H_PRED_CONTOUR) {
       // star-fan triangles: slab order is not the global order
       if ((rc = sort_unique_cells<3>(res.get(), n, S->committed_cells, S->st))) return rc;
@@ -444,8 +529,23 @@ void ftk2h_stream_close(ftk2h_stream* S) {
   S->lambda.release();
   S->parent.release();
   S->cells.release();
+  S->roots.release();
   if (S->st) (void)hipStreamDestroy(S->st);
   delete S;
 }
+
+int ftk2h_stream_set_retire_bytes(ftk2h_stream* S, uint64_t bytes) {
+  if (!S) return fail(FTK2H_E_INVALID, "null argument");
+  S->retire_bytes = bytes;
+  return FTK2H_OK;
+}
+
+uint64_t ftk2h_stream_resident_bytes(const ftk2h_stream* S) {
+  if (!S) return 0;
+  return 2 * S->nsp * S->esz * (uint64_t)S->ncomp + S->keys.bytes + S->lambda.bytes + S->parent.bytes +
+         S->cells.bytes;
+}
+
+uint64_t ftk2h_stream_retired_nodes(const ftk2h_stream* S) { return S ? S->fr.F : 0; }
 
 }  // extern "C"

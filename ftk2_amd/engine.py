@@ -388,10 +388,12 @@ def _shifted(a, shift):
 
 class Stream:
     """Timestep-by-timestep extraction (ftk2h_stream_*): execute_stream semantics with
-    the last timestep and the open track frontier resident on the GPU."""
+    the last timestep and the open track frontier resident on the GPU.  retire_bytes:
+    device payload of finished nodes/lambda/cells beyond which they move to host
+    memory (0: after every slab; None: the library default, 1 GiB)."""
 
     def __init__(self, spatial_dims, kind: int, m: int, dtype: int, ncomp: int, thresholds=(0.0, 0.0),
-                 sos_q: float = 1e6, device: int = 0):
+                 sos_q: float = 1e6, device: int = 0, retire_bytes=None):
         self.spatial = [int(x) for x in spatial_dims]
         pd = _lib.PredicateDesc()
         pd.kind = kind
@@ -406,6 +408,15 @@ class Stream:
                                                 C.byref(h)))
         self.h = h.value
         self.ncomp = ncomp
+        if retire_bytes is not None:
+            _lib.check(_lib.lib().ftk2h_stream_set_retire_bytes(self.h, int(retire_bytes)))
+
+    def resident_bytes(self) -> int:
+        """Device bytes in use (timestep buffers + committed state still on the GPU)."""
+        return int(_lib.lib().ftk2h_stream_resident_bytes(self.h))
+
+    def retired_nodes(self) -> int:
+        return int(_lib.lib().ftk2h_stream_retired_nodes(self.h))
 
     def append(self, comps):
         comps = list(comps)

@@ -191,6 +191,20 @@ int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial
 int ftk2h_stream_append(ftk2h_stream* s, const ftk2h_field_view* comps, void* hip_stream);
 int ftk2h_stream_finish(ftk2h_stream* s, ftk2h_result** out);
 void ftk2h_stream_close(ftk2h_stream* s);
+/* Retirement of finished stream state.  No counterpart in the reference, whose
+ * streaming path copies every slab's result to host memory and links it there
+ * (engine.hpp:756-800).  When the committed nodes, lambda and cells held on the
+ * device exceed `bytes` after a slab, they move to host memory and the device keeps
+ * only the open frontier: parents of the last slab's lower-plane nodes and of the
+ * roots of the tracks that still reach it.  ftk2h_stream_finish returns the same
+ * result either way.  Default 1 GiB; 0 retires after every slab; UINT64_MAX never. */
+int ftk2h_stream_set_retire_bytes(ftk2h_stream* s, uint64_t bytes);
+/* Device bytes in use by the stream: both timestep buffers and the committed
+ * nodes, lambda, cells and union-find parents still on the device (allocations
+ * grow to at most about the retirement threshold plus one slab). */
+uint64_t ftk2h_stream_resident_bytes(const ftk2h_stream* s);
+/* Committed nodes moved to host memory so far. */
+uint64_t ftk2h_stream_retired_nodes(const ftk2h_stream* s);
 
 /* ---- Batching -------------------------------------------------------------------
  * No counterpart in the reference, which runs one SimplicialEngine::execute()

@@ -69,3 +69,54 @@ def test_stream_two_timesteps_and_empty():
     r = s.finish()
     assert r.sizes()[0] == 0
     s.close()
+
+
+def stream_stats(comps, kind, m, thr, retire):
+    nt = comps[0].shape[-1]
+    dt = _lib.F64 if comps[0].dtype == np.float64 else _lib.F32
+    s = engine.Stream(comps[0].shape[:-1], kind, m, dt, len(comps), thr, retire_bytes=retire)
+    resident = []
+    for t in range(nt):
+        s.append([np.asfortranarray(c[..., t]) for c in comps])
+        resident.append(s.resident_bytes())
+    retired = s.retired_nodes()
+    r = s.finish().to_complex()
+    s.close()
+    return r, resident, retired
+
+
+@pytest.mark.parametrize("retire", [0, 4096])
+@pytest.mark.parametrize("case", ["cp2d", "cp3d", "contour", "fiber"])
+def test_stream_retirement_equals_oneshot(case, retire):
+    """Finished nodes, lambda and cells moved to host memory (after every slab, or
+    past a small threshold) give the one-shot complex bit for bit, tracks included
+    (long-lived tracks that merge after their first nodes retired); the device then
+    holds only the timestep buffers and the frontier."""
+    print(f"[stream-retire] {case} {retire}", flush=True)
+    if case == "cp2d":
+        comps, kind, m, thr = list(synth.cp2d_config(nx=40, ny=36, nt=24, seed=12)), 0, 2, (0, 0)
+    elif case == "cp3d":
+        comps, kind, m, thr = synth.cp3d_config(n=16, nt=8, ngauss=6, seed=13, noise=0.02), 0, 3, (0, 0)
+    elif case == "contour":
+        comps = [synth.scalar_config(n=16, nt=10, ngauss=8, seed=14, dtype=np.float64)]
+        kind, m, thr = 1, 1, (0.0, 0)
+    else:
+        comps = [synth.scalar_config(n=14, nt=8, ngauss=8, seed=s, dtype=np.float64) for s in (15, 16)]
+        kind, m, thr = 2, 2, (0.0, 0.0)
+    mesh = engine.RegularSimplicialMesh(comps[0].shape)
+    one = engine.extract(mesh, kind, m, comps, thr).to_complex()
+    assert len(one.vertices) > 50
+    st, resident, retired = stream_stats(comps, kind, m, thr, retire)
+    same(st, one)
+    assert retired > 0
+    # some track spans more than half of the timesteps, so it stayed open across retirements
+    nsp = int(np.prod(comps[0].shape[:-1]))
+    tmin = one.vertices.min(axis=1).astype(np.int64) // nsp
+    span = max(np.ptp(tmin[one.track_id == k]) for k in np.unique(one.track_id))
+    assert span > comps[0].shape[-1] // 2
+    keep, res_keep, ret_keep = stream_stats(comps, kind, m, thr, 2**64 - 1)
+    same(keep, one)
+    assert ret_keep == 0
+    assert resident[-1] < res_keep[-1]
+    if retire == 0:
+        assert max(resident[2:]) < res_keep[-1]
