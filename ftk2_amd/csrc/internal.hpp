@@ -474,8 +474,7 @@ int run_pipeline(RunArgs& A) {
   }
   HIP_TRY(rec(6));
   if (nn) {
-    const uint64_t g = (nn + BLOCK - 1) / BLOCK;
-    k_compress<<<dim3((unsigned)g), dim3(BLOCK), 0, st>>>(parent, R->track, nn);
+    launch_compress(parent, R->track, nn, st);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(rec(7));

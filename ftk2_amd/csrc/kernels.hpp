@@ -1514,16 +1514,38 @@ static __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint32_t
   if (i < n) p[i] = v;
 }
 
+// track[i] = root of i.  The walk is latency-bound, so each thread follows
+// COMPRESS_W independent chains at once (more loads in flight per wave).
+constexpr int COMPRESS_W = 4;
 static __global__ __launch_bounds__(BLOCK) void k_compress(uint32_t* parent, uint32_t* track, uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  uint32_t x = (uint32_t)i;
-  uint32_t p = parent[x];
-  while (p != x) {
-    x = p;
-    p = parent[x];
+  const uint64_t i0 = (uint64_t)blockIdx.x * BLOCK * COMPRESS_W + threadIdx.x;
+  uint32_t x[COMPRESS_W], p[COMPRESS_W];
+#pragma unroll
+  for (int j = 0; j < COMPRESS_W; ++j) {
+    const uint64_t i = i0 + (uint64_t)j * BLOCK;
+    x[j] = (uint32_t)i;
+    p[j] = i < n ? parent[i] : (uint32_t)i;  // beyond n: already "at its root", never loads
   }
-  track[i] = x;
+  while (true) {
+    bool more = false;
+#pragma unroll
+    for (int j = 0; j < COMPRESS_W; ++j)
+      if (p[j] != x[j]) {
+        x[j] = p[j];
+        p[j] = parent[x[j]];
+        more = true;
+      }
+    if (!more) break;
+  }
+#pragma unroll
+  for (int j = 0; j < COMPRESS_W; ++j) {
+    const uint64_t i = i0 + (uint64_t)j * BLOCK;
+    if (i < n) track[i] = x[j];
+  }
+}
+static inline void launch_compress(uint32_t* parent, uint32_t* track, uint64_t n, hipStream_t st) {
+  const uint64_t per = (uint64_t)BLOCK * COMPRESS_W;
+  if (n) k_compress<<<dim3((unsigned)((n + per - 1) / per)), dim3(BLOCK), 0, st>>>(parent, track, n);
 }
 
 }  // namespace ftk2h

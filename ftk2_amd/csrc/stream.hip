@@ -211,8 +211,7 @@ int slot_roots(ftk2h_stream* S, std::vector<uint32_t>& out) {
   if (!P) return FTK2H_OK;
   int rc;
   if ((rc = S->roots.reserve(P * 4, S->st))) return rc;
-  k_compress<<<(unsigned)((P + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(static_cast<uint32_t*>(S->parent.p),
-                                                                       static_cast<uint32_t*>(S->roots.p), P);
+  launch_compress(static_cast<uint32_t*>(S->parent.p), static_cast<uint32_t*>(S->roots.p), P, S->st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out.data(), S->roots.p, P * 4, hipMemcpyDeviceToHost, S->st));
   HIP_TRY(hipStreamSynchronize(S->st));
@@ -493,8 +492,7 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
       HIP_TRY(hipMemcpyAsync(res->track, tracks.data(), n * 4, hipMemcpyHostToDevice, S->st));
       HIP_TRY(hipStreamSynchronize(S->st));  // `tracks` is local
     } else {
-      k_compress<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, S->st>>>(static_cast<uint32_t*>(S->parent.p),
-                                                                           res->track, n);
+      launch_compress(static_cast<uint32_t*>(S->parent.p), res->track, n, S->st);
       HIP_TRY(hipGetLastError());
     }
   }
