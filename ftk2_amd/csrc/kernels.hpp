@@ -28,6 +28,7 @@
 
 #include "certify.hpp"
 #include "exact.hpp"
+#include "filter.hpp"
 #include "tables.hpp"
 
 namespace ftk2h {
@@ -250,24 +251,8 @@ constexpr int tri_index(int a, int b) {
 }
 
 // ---------------------------------------------------------------------------
-// Triple product sign with an error-bounded FP64 filter and an exact fallback.
-// Inputs are quantized integers |q| < 2^20 held exactly in float.
-__device__ __forceinline__ void cross3(const float* A, const float* B, double* e) {
-  // exact: products < 2^40, differences < 2^41
-  e[0] = fma((double)A[1], (double)B[2], -((double)A[2] * (double)B[1]));
-  e[1] = fma((double)A[2], (double)B[0], -((double)A[0] * (double)B[2]));
-  e[2] = fma((double)A[0], (double)B[1], -((double)A[1] * (double)B[0]));
-}
-
-// Filtered sign of e . C where e = A x B (the value det[A; B; C]).  Returns +1/-1
-// when the FP64 value exceeds the rounding-error bound, 0 when it does not (the
-// caller then takes the exact path, which also covers exact zeros).
-__device__ __forceinline__ int dot_sign(const double* e, const float* C, double bnd) {
-  double f = e[0] * (double)C[0];
-  f = fma(e[1], (double)C[1], f);
-  f = fma(e[2], (double)C[2], f);
-  return f > bnd ? 1 : (f < -bnd ? -1 : 0);
-}
+// Triple product signs: the error-bounded FP32 filter of filter.hpp (cross3f,
+// dot_signf); a value within the bound sends its simplex to the exact path.
 
 // Tet types ordered by (b, c, a): tets sharing the pair (b, c) are consecutive, so
 // the cross product Xb x Xc is computed once per pair and dies right after.
@@ -311,7 +296,7 @@ constexpr TetOrder<D> TET_ORDER = build_tet_order<D>();
 // the workgroup's lanes (run_items) so every lane does useful work.
 //
 // Bits: acc = accepted, sol = SoS inside, solver pending, deg = verbatim pending.
-// A triple product whose filtered FP64 value is within its error bound (exact zeros
+// A triple product whose filtered FP32 value is within its error bound (exact zeros
 // included) sends the tet to the verbatim path, which decides it exactly.
 
 struct Masks {
@@ -378,15 +363,15 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
     });
     return Masks{acc, sol, deg};
   }
-  const double bnd = mx * mx * mx * 0x1p-48;
+  const float bnd = (float)(mx * mx * mx) * FTK2H_TP32_BOUND;
   uint64_t tpos = 0, tzero = 0;
   sfor<0, CH<D, 2>.n>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
     // corners outside the grid hold 0 and are masked out by inb
-    double e[3];
-    cross3(X[0], X[a], e);
-    const int sg = dot_sign(e, X[b], bnd);
+    float e[3];
+    cross3f(X[0], X[a], e);
+    const int sg = dot_signf(e, X[b], bnd);
     const uint64_t valid = (inb >> b) & 1;
     tpos |= (valid & (uint64_t)(sg > 0)) << t;
     tzero |= (valid & (uint64_t)(sg == 0)) << t;
@@ -397,9 +382,9 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
   sfor<0, CH<D, 3>.n>([&](auto kk) {
     constexpr int t = TET_ORDER<D>.t[decltype(kk)::value];
     constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
-    double e[3];
-    cross3(X[b], X[c], e);
-    const int sg = dot_sign(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
+    float e[3];
+    cross3f(X[b], X[c], e);
+    const int sg = dot_signf(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
     fpos |= (uint64_t)(sg > 0) << t;
     func |= (uint64_t)(sg == 0) << t;
     __builtin_amdgcn_sched_barrier(0);
