@@ -1117,7 +1117,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_stage2(KParams P) {
 }
 
 template <int KIND, int D, int M, class T>
-__global__ __launch_bounds__(BLOCK, 3) void k_classify(KParams P) {
+__global__ __launch_bounds__(BLOCK, 4) void k_classify(KParams P) {
   classify_tile<KIND, D, M, T>(P, blockIdx.x);
 }
 
