@@ -343,7 +343,7 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
   constexpr int NC = 1 << D;
   const uint32_t inb = corner_mask<D>(P, x.l);
   float X[NC][3];
-  double mx = 0;
+  float mx = 0.f;  // the largest |q|, exact (integers below 2^24 are exact in float)
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
     const bool in = (inb >> s) & 1;
@@ -351,10 +351,10 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       X[s][k] = in ? src[k] : 0.f;
-      mx = fmax(mx, (double)fabsf(X[s][k]));
+      mx = fmaxf(mx, fabsf(X[s][k]));
     }
   }
-  if (!(mx < 0x1p20)) {
+  if (!(mx < 0x1p20f)) {
     // magnitudes beyond the fast filter: every tet takes the verbatim path
     sfor<0, CH<D, 3>.n>([&](auto tt) {
       constexpr int t = decltype(tt)::value;
@@ -363,7 +363,7 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
     });
     return Masks{acc, sol, deg};
   }
-  const float bnd = (float)(mx * mx * mx) * FTK2H_TP32_BOUND;
+  const float bnd = mx * mx * mx * FTK2H_TP32_BOUND;  // mx^3 < 2^60; rounding well inside the 2x margin
   uint64_t tpos = 0, tzero = 0;
   sfor<0, CH<D, 2>.n>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
