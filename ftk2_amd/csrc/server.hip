@@ -3,7 +3,8 @@
 // request is an ftk2h_stream (execute_stream semantics: last timestep and open
 // union-find frontier stay on the device).  A step takes every admitted stream
 // that has a queued timestep, forms its next slab, and extracts the slabs of one
-// predicate template in one batched pass (ftk2h_batch_run); streams join at the
+// predicate template in one batched pass (ftk2h_batch_run), except slabs large
+// enough to fill the device, which run one by one; streams join at the
 // first step after admission and leave when closed and drained.  Because a slab's
 // extraction is independent of the other jobs in its batch, each result equals
 // the one the stream gets alone.
@@ -17,6 +18,11 @@ using namespace ftk2h;
 using namespace ftk2h_internal;
 
 namespace {
+
+// Slabs of at least this many vertices fill the device on their own (8192
+// workgroup tiles and more); they run through the single-mesh pipeline, whose
+// kernels are tuned for one large mesh, instead of the batched pass.
+constexpr uint64_t SOLO_SLAB_VERTICES = (uint64_t)1 << 21;
 
 struct QueuedStep {
   ftk2h_field_view views[3];
@@ -197,6 +203,7 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
     ftk2h_field_view views[3];
   };
   std::map<BatchKey, std::vector<Pending>> groups;
+  std::vector<Pending> solo;
   for (auto& [id, S] : s->streams) {
     if (S.finished) continue;
     while (!S.queue.empty() && !stream_slab_ready(S.s)) {
@@ -209,12 +216,25 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
       p.S = &S;
       std::memset(&p.job, 0, sizeof(p.job));
       stream_slab_desc(S.s, &p.job.mesh, p.views);
-      groups[key_of(S.pred, S.d)].push_back(p);
+      uint64_t nv = 1;
+      for (int j = 0; j < p.job.mesh.ndims; ++j) nv *= p.job.mesh.local_dims[j];
+      if (nv >= SOLO_SLAB_VERTICES)
+        solo.push_back(p);
+      else
+        groups[key_of(S.pred, S.d)].push_back(p);
     }
   }
   // the timestep copies were issued on each stream's own HIP stream
   HIP_TRY(hipDeviceSynchronize());
-  // 2. one batched pass per predicate template
+  // 2a. slabs that fill the device on their own: the single-mesh pipeline, one by one
+  for (Pending& p : solo) {
+    ftk2h_result* R = nullptr;
+    int rc = ftk2h_extract_regular(s->ctx, &p.job.mesh, &p.S->pred, p.views, p.S->ncomp, nullptr, nullptr, &R,
+                                   s->st);
+    if (!rc) rc = stream_take_slab(p.S->s, R);
+    if (rc) return rc;
+  }
+  // 2b. one batched pass per predicate template for the others
   for (auto& [key, list] : groups) {
     ftk2h_batch*& b = s->batches[key];
     if (!b) {

@@ -137,3 +137,27 @@ def test_server_rejects_and_reports():
     r = srv.take(sid).to_complex()
     ref = oracle.run([u, v], oracle.PRED_CP, 2)
     assert np.array_equal(r.vertices, ref["verts"])
+
+
+def test_device_filling_slabs_run_alone_in_the_mix():
+    """Slabs of 2^21 vertices and more take the single-mesh pipeline inside a step
+    while the small ones of the same step are batched; every result still equals
+    the request processed alone, and the big request equals its one-shot
+    extraction."""
+    reqs = mixed_requests(6, seed=4)
+    big = synth.gaussians((1100, 1000), 3, 12, seed=44)  # slab: 1100 x 1000 x 2 > 2^21 vertices
+    reqs.append(serve.Request(len(reqs), 0.0, (1100, 1000), _lib.PRED_CRITICAL_POINT, 2, _lib.F64,
+                              [[c[..., t] for c in big] for t in range(3)], simplices=1099 * 999 * 2 * 12,
+                              meta={"comps": big}))
+    srv = engine.Server(64 << 30)
+    outs, wall, rej = serve.run(srv, reqs, keep_results=True)
+    assert rej == [] and len(outs) == len(reqs)
+    for o in outs:
+        rq = reqs[o.rid]
+        say(f"[serve] request {o.rid}: spatial={rq.spatial} T={len(rq.timesteps)} m={rq.m}")
+        got = o.result.to_complex()
+        same(got, alone(rq), f"request {o.rid}")
+        if o.rid == len(reqs) - 1:
+            assert len(got.vertices) > 100
+            one = engine.extract(engine.RegularSimplicialMesh((1100, 1000, 3)), 0, 2, list(big)).to_complex()
+            same(got, one, "big request vs one-shot")
