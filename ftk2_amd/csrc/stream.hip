@@ -135,7 +135,7 @@ struct ftk2h_stream {
   DevVec keys, lambda, parent, cells, roots;
   ftk2h_frontier::Frontier fr;
   uint64_t retired_cells = 0;
-  uint64_t retire_bytes = (uint64_t)1 << 30;  // device payload that triggers retirement
+  uint64_t retire_bytes = (uint64_t)8 << 30;  // device payload that triggers retirement
   std::vector<uint64_t> h_keys;               // retired payload, in committed order
   std::vector<char> h_lambda;
   std::vector<uint32_t> h_cells;

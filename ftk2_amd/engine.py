@@ -390,7 +390,7 @@ class Stream:
     """Timestep-by-timestep extraction (ftk2h_stream_*): execute_stream semantics with
     the last timestep and the open track frontier resident on the GPU.  retire_bytes:
     device payload of finished nodes/lambda/cells beyond which they move to host
-    memory (0: after every slab; None: the library default, 1 GiB)."""
+    memory (0: after every slab; None: the library default, 8 GiB)."""
 
     def __init__(self, spatial_dims, kind: int, m: int, dtype: int, ncomp: int, thresholds=(0.0, 0.0),
                  sos_q: float = 1e6, device: int = 0, retire_bytes=None):

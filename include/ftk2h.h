@@ -197,7 +197,9 @@ void ftk2h_stream_close(ftk2h_stream* s);
  * device exceed `bytes` after a slab, they move to host memory and the device keeps
  * only the open frontier: parents of the last slab's lower-plane nodes and of the
  * roots of the tracks that still reach it.  ftk2h_stream_finish returns the same
- * result either way.  Default 1 GiB; 0 retires after every slab; UINT64_MAX never. */
+ * result either way.  Default 8 GiB; 0 retires after every slab; UINT64_MAX never.
+ * Retirement copies through pageable host memory and waits for the copy, so a low
+ * threshold trades append time for device memory. */
 int ftk2h_stream_set_retire_bytes(ftk2h_stream* s, uint64_t bytes);
 /* Device bytes in use by the stream: both timestep buffers and the committed
  * nodes, lambda, cells and union-find parents still on the device (allocations
