@@ -152,6 +152,14 @@ class SimplicialEngine {
     }
   }
 
+  // Device payload of finished nodes, lambda and cells beyond which execute_stream
+  // moves them to host memory (ftk2h_stream_set_retire_bytes; no counterpart in the
+  // reference, which keeps every slab's result in host memory).
+  void set_stream_retire_bytes(uint64_t bytes) {
+    retire_bytes_ = bytes;
+    retire_set_ = true;
+  }
+
   // engine.hpp:214-270 — timesteps read one at a time; each slab [t, t+1] is
   // extracted on the device as it arrives (ftk2h_stream_*), the last timestep and
   // the open union-find frontier stay resident.  The result equals execute() over
@@ -216,6 +224,7 @@ class SimplicialEngine {
           if (spatial.empty() || spatial.size() > 3) throw std::runtime_error("execute_stream: 1-3 spatial dims");
           detail::check(ftk2h_stream_open(detail::context(device_), (int)spatial.size(), spatial.data(), &pd,
                                           (int)names.size(), &st));
+          if (retire_set_) detail::check(ftk2h_stream_set_retire_bytes(st, retire_bytes_));
         }
         detail::check(ftk2h_stream_append(st, views.data(), nullptr));
       }
@@ -361,6 +370,8 @@ class SimplicialEngine {
   std::shared_ptr<Mesh> mesh_;
   PredicateType predicate_;
   int device_ = 0;
+  uint64_t retire_bytes_ = 0;
+  bool retire_set_ = false;
   ftk2h_result* result_ = nullptr;
   uint64_t tstride_ = 0;  // id step of the layer axis when mesh_ is extruded
 };

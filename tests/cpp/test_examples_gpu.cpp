@@ -156,6 +156,21 @@ static void streaming() {
     CHECK_EQ(streamed.connectivity[c].indices == one.connectivity[c].indices, true);
   check_complex(streamed);
   std::printf("streaming: %zu vertices streamed, %zu one-shot\n", streamed.vertices.size(), one.vertices.size());
+
+  // the same stream with finished state moved to host memory after every slab
+  engine.set_stream_retire_bytes(0);
+  engine.execute_stream(stream);
+  auto retired = engine.get_complex();
+  CHECK_EQ(retired.vertices.size(), one.vertices.size());
+  for (size_t i = 0; i < one.vertices.size() && i < retired.vertices.size(); ++i) {
+    CHECK_EQ(retired.vertices[i].simplex < one.vertices[i].simplex, false);
+    CHECK_EQ(one.vertices[i].simplex < retired.vertices[i].simplex, false);
+    CHECK_EQ(retired.vertices[i].track_id, one.vertices[i].track_id);
+  }
+  CHECK_EQ(retired.connectivity.size(), one.connectivity.size());
+  for (size_t c = 0; c < one.connectivity.size() && c < retired.connectivity.size(); ++c)
+    CHECK_EQ(retired.connectivity[c].indices == one.connectivity[c].indices, true);
+  std::printf("streaming_retired: %zu vertices\n", retired.vertices.size());
 }
 
 // Attribute recording (engine.hpp:1144-1201) through the engine's AttributeSpec

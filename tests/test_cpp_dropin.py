@@ -39,5 +39,6 @@ def test_cpp_examples_on_gpu():
     assert p.returncode == 0, p.stdout + p.stderr
     assert "66418" in p.stdout and "6078" in p.stdout
     assert "streaming:" in p.stdout  # execute_stream == execute through ftk::stream<>
+    assert "streaming_retired:" in p.stdout  # the same with retirement after every slab
     assert "attributes_3d:" in p.stdout
     assert "extruded_2d:" in p.stdout  # ExtrudedSimplicialMesh over a regular base
