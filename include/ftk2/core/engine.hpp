@@ -168,10 +168,6 @@ class SimplicialEngine {
     (void)var_names;  // the predicate's own variable names select the arrays
     const int n = s.total_timesteps();
     if (n < 2) return;  // as the reference: nothing extracted, earlier results kept
-    if constexpr (detail::pred_traits<PredicateType>::supported) {
-      if (!predicate_.attributes.empty())
-        throw std::runtime_error("SimplicialEngine: attribute recording is not supported by execute_stream");
-    }
     clear_results();
     if constexpr (!detail::pred_traits<PredicateType>::supported) {
       throw std::runtime_error("SimplicialEngine: predicate not supported on the GPU");
@@ -193,6 +189,29 @@ class SimplicialEngine {
           pd.thresholds[j] = (double)predicate_.thresholds[j];
         }
       }
+      // Attribute recording (engine.hpp:1144-1201) as the reference's slabs allow it:
+      // a slab holds only the arrays the predicate reads, plus a CP scalar variable
+      // (engine.hpp:218-228), so specs on other sources are skipped, and the values
+      // come through get_value at (global - slab offset) coordinates, i.e. each
+      // vertex's value at its own timestep.  Those sources' timesteps are kept in
+      // host memory and interpolated over the whole spacetime grid after the last
+      // slab, which gives the same floats as the per-slab recording.
+      std::vector<std::string> slab_vars = names;
+      if constexpr (detail::pred_traits<PredicateType>::kind == FTK2H_PRED_CRITICAL_POINT)
+        if (!predicate_.scalar_var_name.empty()) slab_vars.push_back(predicate_.scalar_var_name);
+      std::vector<const AttributeSpec*> specs;
+      std::vector<size_t> spec_src;  // index into slab_vars
+      std::vector<char> src_used(slab_vars.size(), 0);
+      for (const auto& spec : predicate_.attributes) {
+        if (spec.slot < 0 || spec.slot >= 16) continue;
+        size_t k = 0;
+        while (k < slab_vars.size() && slab_vars[k] != spec.source) ++k;
+        if (k == slab_vars.size()) continue;  // not in the slab data: skipped (engine.hpp:1148-1149)
+        specs.push_back(&spec);
+        spec_src.push_back(k);
+        src_used[k] = 1;
+      }
+      std::vector<std::vector<T>> src_steps(slab_vars.size());
       std::vector<uint64_t> spatial;
       ftk2h_stream* st = nullptr;
       struct Closer {
@@ -222,6 +241,24 @@ class SimplicialEngine {
         }
         if (t == 0) {
           if (spatial.empty() || spatial.size() > 3) throw std::runtime_error("execute_stream: 1-3 spatial dims");
+          // A slab array whose first extent is 1..16 counts as multi-component
+          // (engine.hpp:1162); a component or magnitude spec then reads it at global
+          // coordinates (engine.hpp:1164-1185), which lie outside a two-timestep slab.
+          const bool multi = spatial[0] >= 1 && spatial[0] <= 16;
+          for (const auto* spec : specs)
+            if (multi && (spec->component >= 0 || spec->type == "magnitude"))
+              throw std::runtime_error("execute_stream: attribute '" + spec->source +
+                                       "' would read a slab as a multi-component array at global coordinates");
+        }
+        for (size_t k = 0; k < slab_vars.size(); ++k) {
+          if (!src_used[k]) continue;
+          const auto& a = group->template get_ref<T>(slab_vars[k]);
+          size_t nsp = 1;
+          for (auto e : spatial) nsp *= (size_t)e;
+          if (a.nelem() != nsp) throw std::runtime_error("execute_stream: attribute source differs in shape");
+          src_steps[k].insert(src_steps[k].end(), a.data(), a.data() + nsp);
+        }
+        if (t == 0) {
           detail::check(ftk2h_stream_open(detail::context(device_), (int)spatial.size(), spatial.data(), &pd,
                                           (int)names.size(), &st));
           if (retire_set_) detail::check(ftk2h_stream_set_retire_bytes(st, retire_bytes_));
@@ -229,6 +266,31 @@ class SimplicialEngine {
         detail::check(ftk2h_stream_append(st, views.data(), nullptr));
       }
       detail::check(ftk2h_stream_finish(st, &result_));
+      if (!specs.empty()) {
+        const int d = (int)spatial.size() + 1;
+        ftk2h_mesh_desc md{};  // the whole spacetime grid
+        md.ndims = d;
+        for (int j = 0; j < 4; ++j) {
+          md.local_dims[j] = j < d - 1 ? spatial[j] : (j == d - 1 ? (uint64_t)n : 1);
+          md.global_dims[j] = md.local_dims[j];
+        }
+        std::vector<ftk2h_attribute_desc> ad;
+        for (size_t i = 0; i < specs.size(); ++i) {
+          ftk2h_attribute_desc a{};
+          a.slot = specs[i]->slot;
+          a.ncomp = 1;
+          a.comps[0].ptr = src_steps[spec_src[i]].data();
+          int64_t str = 1;
+          for (int j = 0; j < d; ++j) {
+            a.comps[0].strides[j] = str;
+            str *= (int64_t)md.local_dims[j];
+          }
+          a.comps[0].on_device = 0;
+          ad.push_back(a);
+        }
+        detail::check(ftk2h_result_add_attributes(detail::context(device_), result_, &md, ad.data(), (int)ad.size(),
+                                                  nullptr));
+      }
     }
   }
 

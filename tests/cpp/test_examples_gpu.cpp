@@ -171,6 +171,22 @@ static void streaming() {
   for (size_t c = 0; c < one.connectivity.size() && c < retired.connectivity.size(); ++c)
     CHECK_EQ(retired.connectivity[c].indices == one.connectivity[c].indices, true);
   std::printf("streaming_retired: %zu vertices\n", retired.vertices.size());
+
+  // A first extent of 16 makes the reference read each slab as a multi-component
+  // array at global coordinates (engine.hpp:1162-1185), outside the slab: refused.
+  AttributeSpec comp0;
+  comp0.source = "scalar";
+  comp0.component = 0;
+  comp0.slot = 0;
+  pred.attributes = {comp0};
+  SimplicialEngine<double, ContourPredicate<double>> eng3(mesh, pred);
+  bool threw = false;
+  try {
+    eng3.execute_stream(stream);
+  } catch (const std::runtime_error&) {
+    threw = true;
+  }
+  CHECK_EQ(threw, true);
 }
 
 // Attribute recording (engine.hpp:1144-1201) through the engine's AttributeSpec
@@ -229,6 +245,43 @@ static void attributes_3d() {
     CHECK_EQ(std::fabs(el.attributes[9] - px) < 1e-4, true);
   }
   std::printf("attributes_3d: %zu vertices\n", cx.vertices.size());
+
+  // Streamed (engine.hpp:214-270): the slabs hold only the predicate's arrays, so
+  // the specs on U and W are recorded (the same floats as the one-shot pass) and
+  // those on X, VEC and a missing source stay 0.
+  AttributeSpec s5; s5.source = "U"; s5.slot = 3;
+  AttributeSpec s6; s6.source = "W"; s6.slot = 9;  // overwrites the X spec's slot
+  pred.attributes = {s0, s1, s2, s3, s4, s5, s6};
+  SimplicialEngine<double, CriticalPointPredicate<3, double>> eng2(mesh, pred);
+  eng2.execute(data);
+  auto one = eng2.get_complex();
+  ftk::stream<> stream(Tn, [&](int t) {
+    auto g = std::make_shared<ftk::ndarray_group>();
+    const char* names[4] = {"U", "V", "W", "X"};
+    const ftk::ndarray<double>* src[4] = {&u, &v, &w, &xs};
+    for (int k = 0; k < 4; ++k) {
+      ftk::ndarray<double> a({(size_t)W, (size_t)H, (size_t)Dd});
+      for (int z = 0; z < Dd; ++z)
+        for (int y = 0; y < H; ++y)
+          for (int x = 0; x < W; ++x) a.f(x, y, z) = src[k]->f(x, y, z, t);
+      g->set(names[k], a);
+    }
+    return g;
+  });
+  eng2.set_stream_retire_bytes(0);  // finished nodes move to host memory after every slab
+  eng2.execute_stream(stream);
+  auto streamed = eng2.get_complex();
+  CHECK_EQ(streamed.vertices.size(), one.vertices.size());
+  for (size_t i = 0; i < one.vertices.size() && i < streamed.vertices.size(); ++i) {
+    const auto& a = streamed.vertices[i];
+    const auto& b = one.vertices[i];
+    CHECK_EQ(a.simplex < b.simplex || b.simplex < a.simplex, false);
+    for (int k = 0; k < 16; ++k) {
+      const bool slab_source = k == 0 || k == 3 || k == 9;
+      CHECK_EQ(a.attributes[k] == (slab_source ? b.attributes[k] : 0.0f), true);
+    }
+  }
+  std::printf("attributes_stream: %zu vertices\n", streamed.vertices.size());
 }
 
 // An ExtrudedSimplicialMesh over a regular 2-D base (what the reference's
