@@ -113,6 +113,45 @@ static void critical_point_3d() {  // test_examples.cpp:153-186 (see tests/test_
   std::printf("critical_point_3d: %zu vertices\n", cx.vertices.size());
 }
 
+// test_examples.cpp:39-105: CP2D on an offset mesh with a scalar variable.  The
+// reference reads its field from hguo/ndarray's synthetic "woven" stream (delta =
+// 1/(DT-1)); the woven function restated here is FTK's published one,
+// cos(x cos t - y sin t) sin(x sin t + y cos t) on [-7.5, 7.5]^2.
+static void critical_point_2d() {
+  const int DW = 16, DH = 16, DT = 5;
+  ftk::ndarray<double> scalar({(size_t)DW, (size_t)DH, (size_t)DT});
+  for (int t = 0; t < DT; ++t)
+    for (int y = 0; y < DH; ++y)
+      for (int x = 0; x < DW; ++x) {
+        const double tt = t * (1.0 / (DT - 1));
+        const double X = ((double)x / (DW - 1) - 0.5) * 15.0, Y = ((double)y / (DH - 1) - 0.5) * 15.0;
+        scalar.f(x, y, t) = std::cos(X * std::cos(tt) - Y * std::sin(tt)) * std::sin(X * std::sin(tt) + Y * std::cos(tt));
+      }
+  ftk::ndarray<double> u({(size_t)DW, (size_t)DH, (size_t)DT}), v({(size_t)DW, (size_t)DH, (size_t)DT});
+  u.fill(0.0);
+  v.fill(0.0);
+  for (int t = 0; t < DT; ++t)
+    for (int y = 1; y < DH - 1; ++y)
+      for (int x = 1; x < DW - 1; ++x) {
+        u.f(x, y, t) = (scalar.f(x + 1, y, t) - scalar.f(x - 1, y, t)) / 2.0;
+        v.f(x, y, t) = (scalar.f(x, y + 1, t) - scalar.f(x, y - 1, t)) / 2.0;
+      }
+  std::map<std::string, ftk::ndarray<double>> data = {{"U", u}, {"V", v}, {"Woven", scalar}};
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{(uint64_t)DW - 2, (uint64_t)DH - 2, (uint64_t)DT},
+                                                      std::vector<uint64_t>{1, 1, 0},
+                                                      std::vector<uint64_t>{(uint64_t)DW, (uint64_t)DH, (uint64_t)DT});
+  CriticalPointPredicate<2, double> cp_pred;
+  cp_pred.var_names[0] = "U";
+  cp_pred.var_names[1] = "V";
+  cp_pred.scalar_var_name = "Woven";
+  SimplicialEngine<double, CriticalPointPredicate<2, double>> engine(mesh, cp_pred);
+  engine.execute(data, {"U", "V", "Woven"});
+  auto cx = engine.get_complex();
+  CHECK_EQ(cx.vertices.size(), (size_t)788);
+  check_complex(cx);
+  std::printf("critical_point_2d: %zu vertices\n", cx.vertices.size());
+}
+
 // test_examples.cpp:259-297: streamed execution equals one-shot execution.  The
 // reference builds its field with hguo/ndarray's synthetic_woven_2Dt (not available
 // here); this uses a woven-like analytic field of the same shape and threshold.
@@ -333,6 +372,7 @@ int main() {
   levelset_3d();
   fiber_3d();
   critical_point_3d();
+  critical_point_2d();
   streaming();
   std::printf("%d failures\n", failures);
   return failures;

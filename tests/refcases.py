@@ -53,6 +53,32 @@ def levelset3d_float(DW=32, DH=32, DD=32, DT=10):
     return [np.asfortranarray(np.broadcast_to(s, (DW, DH, DD, DT)), dtype=np.float32)]
 
 
+def woven_cp2d(DW=16, DH=16, DT=5):
+    """test_examples.cpp:39-105 — CP2D on the gradient of the "woven" synthetic field,
+    mesh of local dims (DW-2, DH-2, DT) at offset (1, 1, 0) in a (DW, DH, DT) grid,
+    scalar variable "Woven"; golden: 788 vertices.
+
+    The field comes from hguo/ndarray's synthetic stream (`format: synthetic`, name
+    woven, `delta` = 1/(DT-1); the library is not vendored and its version is not
+    pinned).  Restated from FTK's published woven function:
+    woven(x, y, t) = cos(x cos t - y sin t) * sin(x sin t + y cos t) with
+    x = (i/(DW-1) - 0.5) * 15, y = (j/(DH-1) - 0.5) * 15 and t = k * delta.  The
+    gradient is the test's own central difference over the interior (0 on the
+    border).  Returns (u, v, scalar), each indexed [x, y, t]."""
+    i = np.arange(DW, dtype=np.float64)[:, None, None]
+    j = np.arange(DH, dtype=np.float64)[None, :, None]
+    t = np.arange(DT, dtype=np.float64)[None, None, :] * (1.0 / (DT - 1))
+    x = (i / (DW - 1) - 0.5) * 15.0
+    y = (j / (DH - 1) - 0.5) * 15.0
+    s = np.cos(x * np.cos(t) - y * np.sin(t)) * np.sin(x * np.sin(t) + y * np.cos(t))
+    s = np.asfortranarray(s)
+    u = np.zeros_like(s, order="F")
+    v = np.zeros_like(s, order="F")
+    u[1:-1, 1:-1, :] = (s[2:, 1:-1, :] - s[:-2, 1:-1, :]) / 2.0
+    v[1:-1, 1:-1, :] = (s[1:-1, 2:, :] - s[1:-1, :-2, :]) / 2.0
+    return u, v, s
+
+
 def cp1d_engine():
     """test_engine.cpp:36-51 — u(x,0) = x - 3.5, u(x,1) = x - 4.5 on a 10x2 mesh."""
     u = np.zeros((10, 2), order="F")

@@ -37,7 +37,7 @@ def test_cpp_examples_on_gpu():
     p = subprocess.run([BIN], capture_output=True, text=True, timeout=110)
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
-    assert "66418" in p.stdout and "6078" in p.stdout
+    assert "66418" in p.stdout and "6078" in p.stdout and "critical_point_2d: 788" in p.stdout
     assert "streaming:" in p.stdout  # execute_stream == execute through ftk::stream<>
     assert "streaming_retired:" in p.stdout  # the same with retirement after every slab
     assert "attributes_3d:" in p.stdout

@@ -59,6 +59,26 @@ def test_levelset3d_reference_case():
     assert len(cx.vertices) == 66418
 
 
+def test_cp2d_woven_reference_case():
+    """test_examples.cpp:39-105: offset mesh, separate U/V arrays, scalar variable."""
+    u, v, s = refcases.woven_cp2d()
+    say("[parity] cp2d_woven_16^2x5 offset (1,1,0)")
+    mesh = engine.RegularSimplicialMesh([14, 14, 5], [1, 1, 0], [16, 16, 5])
+    pred = engine.CriticalPointPredicate(M=2, var_names=["U", "V"], scalar_var_name="Woven")
+    eng = engine.SimplicialEngine(mesh, pred)
+    eng.execute({"U": u, "V": v, "Woven": s})
+    cx = eng.get_complex()
+    ref = oracle.run([u, v], oracle.PRED_CP, 2, dims=(14, 14, 5), offset=(1, 1, 0), gdims=(16, 16, 5), scalar=s)
+    assert len(cx.vertices) == 788
+    assert np.array_equal(cx.vertices, ref["verts"])
+    assert np.array_equal(cx.lam.view(np.uint64), ref["lam"].view(np.uint64))
+    assert np.array_equal(cx.scalar, ref["scalar"])
+    assert np.array_equal(cx.track_id, ref["track"])
+    assert sorted(cx.connectivity) == sorted(ref["conn"])
+    for dim, cells in ref["conn"].items():
+        assert np.array_equal(cx.connectivity[dim], cells)
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_cp2d_config0(seed):
     u, v = synth.cp2d_config(seed=seed)

@@ -19,6 +19,23 @@ def test_fiber3d_golden_6078():  # test_examples.cpp:188-227
     assert len(r["verts"]) == 6078
 
 
+def test_cp2d_woven_golden_788():  # test_examples.cpp:39-105
+    u, v, s = refcases.woven_cp2d()
+    r = oracle.run([u, v], oracle.PRED_CP, 2, dims=(14, 14, 5), offset=(1, 1, 0), gdims=(16, 16, 5), scalar=s)
+    assert len(r["verts"]) == 788
+    # the time parametrisation matters: integer times give another complex
+    tt = np.arange(5, dtype=np.float64)[None, None, :]
+    x = (np.arange(16)[:, None, None] / 15 - 0.5) * 15.0
+    y = (np.arange(16)[None, :, None] / 15 - 0.5) * 15.0
+    s2 = np.asfortranarray(np.cos(x * np.cos(tt) - y * np.sin(tt)) * np.sin(x * np.sin(tt) + y * np.cos(tt)))
+    u2 = np.zeros_like(s2, order="F")
+    v2 = np.zeros_like(s2, order="F")
+    u2[1:-1, 1:-1, :] = (s2[2:, 1:-1, :] - s2[:-2, 1:-1, :]) / 2.0
+    v2[1:-1, 1:-1, :] = (s2[1:-1, 2:, :] - s2[1:-1, :-2, :]) / 2.0
+    r2 = oracle.run([u2, v2], oracle.PRED_CP, 2, dims=(14, 14, 5), offset=(1, 1, 0), gdims=(16, 16, 5))
+    assert len(r2["verts"]) != 788
+
+
 def test_cp3d_linear_float():
     """test_examples.cpp:153-186 asserts 52 vertices.  A literal transcription of the
     reference sources (sos.hpp det3 with its explicit 4x4 cofactor formula,
