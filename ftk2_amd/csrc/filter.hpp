@@ -28,10 +28,16 @@ FTK2H_FHD void cross3f(const float* A, const float* B, float* e) {
   e[2] = fmaf(A[0], B[1], -(A[1] * B[0]));
 }
 
-// +1 / -1 when (e . C) is certainly positive / negative, 0 when |e . C| <= bnd.
-FTK2H_FHD int dot_signf(const float* e, const float* C, float bnd) {
+// e . C with the rounding the bound above assumes.
+FTK2H_FHD float dot3f(const float* e, const float* C) {
   float f = e[0] * C[0];
   f = fmaf(e[1], C[1], f);
   f = fmaf(e[2], C[2], f);
+  return f;
+}
+
+// +1 / -1 when (e . C) is certainly positive / negative, 0 when |e . C| <= bnd.
+FTK2H_FHD int dot_signf(const float* e, const float* C, float bnd) {
+  const float f = dot3f(e, C);
   return f > bnd ? 1 : (f < -bnd ? -1 : 0);
 }

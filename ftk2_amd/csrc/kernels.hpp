@@ -335,6 +335,16 @@ __device__ __forceinline__ void stage_quantized3(const KParams& P, int64_t v0, f
   }
 }
 
+// Bit i of a sequence of N flags shifted into 32-bit words one at a time
+// (w = (w << 1) | flag, flags 32j.. in word j): the last flag of a word is bit 0.
+template <int N, int I>
+__device__ __forceinline__ uint32_t pushed_bit(const uint32_t* w) {
+  constexpr int word = I >> 5;
+  constexpr int in_word = (N - 32 * word) < 32 ? (N - 32 * word) : 32;
+  constexpr int pos = in_word - 1 - (I & 31);
+  return (w[word] >> pos) & 1u;
+}
+
 // Critical points of a 3-component field on 3-simplices (CriticalPointPredicate<3,T>,
 // predicate.hpp:66-99).
 template <class T, int D>
@@ -364,46 +374,55 @@ __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, co
     return Masks{acc, sol, deg};
   }
   const float bnd = mx * mx * mx * FTK2H_TP32_BOUND;  // mx^3 < 2^60; rounding well inside the 2x margin
-  uint64_t tpos = 0, tzero = 0;
-  sfor<0, CH<D, 2>.n>([&](auto tt) {
+  // Each triple product f gives two bits, shifted into 32-bit words in the order
+  // computed (one v_alignbit each): its sign bit (the exact sign when f is certain)
+  // and "certain" = the sign bit of bnd - |f|, set iff |f| > bnd (a difference of
+  // two floats is zero only when they are equal and keeps its sign when flushed).
+  // Corners outside the grid hold 0; a tet whose last corner c is inside has all its
+  // corners inside (they are subsets of c), so only the tet masks need the inb gate.
+  constexpr int NTRI = CH<D, 2>.n, NTET = CH<D, 3>.n;
+  uint32_t tn[2] = {0, 0}, tc[2] = {0, 0};  // triangles (0, a, b), by chain type
+  sfor<0, NTRI>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
-    // corners outside the grid hold 0 and are masked out by inb
     float e[3];
     cross3f(X[0], X[a], e);
-    const int sg = dot_signf(e, X[b], bnd);
-    const uint64_t valid = (inb >> b) & 1;
-    tpos |= (valid & (uint64_t)(sg > 0)) << t;
-    tzero |= (valid & (uint64_t)(sg == 0)) << t;
+    const float f = dot3f(e, X[b]);
+    tn[t >> 5] = __builtin_amdgcn_alignbit(tn[t >> 5], __float_as_uint(f), 31);
+    tc[t >> 5] = __builtin_amdgcn_alignbit(tc[t >> 5], __float_as_uint(bnd - fabsf(f)), 31);
     __builtin_amdgcn_sched_barrier(0);  // keep live ranges short
   });
-  // signs of the opposite faces tp(a,b,c), in (b,c)-grouped order
-  uint64_t fpos = 0, func = 0;
-  sfor<0, CH<D, 3>.n>([&](auto kk) {
-    constexpr int t = TET_ORDER<D>.t[decltype(kk)::value];
+  // opposite faces tp(a,b,c), in (b,c)-grouped order k
+  uint32_t fn[2] = {0, 0}, fc[2] = {0, 0};
+  sfor<0, NTET>([&](auto kk) {
+    constexpr int k = decltype(kk)::value;
+    constexpr int t = TET_ORDER<D>.t[k];
     constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
     float e[3];
     cross3f(X[b], X[c], e);
-    const int sg = dot_signf(e, X[a], bnd);  // tp(a,b,c) = (Xb x Xc) . Xa
-    fpos |= (uint64_t)(sg > 0) << t;
-    func |= (uint64_t)(sg == 0) << t;
+    const float f = dot3f(e, X[a]);  // tp(a,b,c) = (Xb x Xc) . Xa
+    fn[k >> 5] = __builtin_amdgcn_alignbit(fn[k >> 5], __float_as_uint(f), 31);
+    fc[k >> 5] = __builtin_amdgcn_alignbit(fc[k >> 5], __float_as_uint(bnd - fabsf(f)), 31);
     __builtin_amdgcn_sched_barrier(0);
   });
   // d3 = det3(v0,v1,v2) = tp(0ab); d2 = det3(v0,v3,v1) = -tp(0ac);
   // d1 = det3(v0,v2,v3) = tp(0bc); d0 = det3(v1,v3,v2) = -tp(abc).
   // Inside by level 0 iff all four are equal; any uncertain/zero value -> verbatim.
-  sfor<0, CH<D, 3>.n>([&](auto tt) {
-    constexpr int t = decltype(tt)::value;
+  // With all four certain, each comparison below is an XOR of two sign bits.
+  sfor<0, NTET>([&](auto kk) {
+    constexpr int k = decltype(kk)::value;
+    constexpr int t = TET_ORDER<D>.t[k];
     constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
     constexpr int i012 = tri_index<D>(a, b), i013 = tri_index<D>(a, c), i023 = tri_index<D>(b, c);
     const uint64_t valid = (inb >> c) & 1;
-    const uint64_t z = ((tzero >> i012) | (tzero >> i013) | (tzero >> i023) | (func >> t)) & 1;
-    const uint64_t p012 = (tpos >> i012) & 1, p013 = (tpos >> i013) & 1, p023 = (tpos >> i023) & 1;
-    const uint64_t pabc = (fpos >> t) & 1;
-    // d3 == d2 <=> p012 != p013;  d1 == d3 <=> p023 == p012;  d0 == d3 <=> pabc != p012
-    const uint64_t in = (p012 ^ p013) & ~(p023 ^ p012) & (pabc ^ p012) & 1;
-    deg |= (valid & z) << t;
-    sol |= (valid & ~z & in & 1) << t;
+    const uint32_t cert = pushed_bit<NTRI, i012>(tc) & pushed_bit<NTRI, i013>(tc) & pushed_bit<NTRI, i023>(tc) &
+                          pushed_bit<NTET, k>(fc);
+    const uint32_t n012 = pushed_bit<NTRI, i012>(tn), n013 = pushed_bit<NTRI, i013>(tn);
+    const uint32_t n023 = pushed_bit<NTRI, i023>(tn), nabc = pushed_bit<NTET, k>(fn);
+    // d3 == d2 <=> n012 != n013;  d1 == d3 <=> n023 == n012;  d0 == d3 <=> nabc != n012
+    const uint32_t in = (n012 ^ n013) & ~(n023 ^ n012) & (nabc ^ n012) & 1;
+    deg |= (valid & (uint64_t)(~cert & 1)) << t;
+    sol |= (valid & (uint64_t)(cert & in)) << t;
   });
   return Masks{acc, sol, 0, deg};  // the verbatim levels run from the tile
 }

@@ -152,14 +152,14 @@ static void critical_point_2d() {
   std::printf("critical_point_2d: %zu vertices\n", cx.vertices.size());
 }
 
-// test_examples.cpp:259-297: streamed execution equals one-shot execution.  The
-// reference builds its field with hguo/ndarray's synthetic_woven_2Dt (not available
-// here); this uses a woven-like analytic field of the same shape and threshold.
+// test_examples.cpp:259-297: streamed execution equals one-shot execution, on the
+// woven field of critical_point_2d() (t = k / (Tn - 1)) at contour level 0.5.
 static void streaming() {
   const int W = 16, H = 16, Tn = 5;
   auto value = [](int x, int y, int t) {
-    const double s = (double)t / (Tn - 1);
-    return std::cos(0.4 * x + s) * std::sin(0.35 * y - s) + 0.25 * std::sin(0.7 * (x + y) * s);
+    const double tt = t * (1.0 / (Tn - 1));
+    const double X = ((double)x / (W - 1) - 0.5) * 15.0, Y = ((double)y / (H - 1) - 0.5) * 15.0;
+    return std::cos(X * std::cos(tt) - Y * std::sin(tt)) * std::sin(X * std::sin(tt) + Y * std::cos(tt));
   };
   ftk::ndarray<double> scalar({(size_t)W, (size_t)H, (size_t)Tn});
   for (int t = 0; t < Tn; ++t)
