@@ -335,6 +335,33 @@ __device__ __forceinline__ void stage_quantized3(const KParams& P, int64_t v0, f
   }
 }
 
+// The same tile for the det2 predicates (CriticalPointPredicate<2,T>, and
+// FiberPredicate<T> on values minus thresholds): two quantized values per vertex as
+// exact integer-valued doubles, each corner value loaded and quantized once per
+// workgroup instead of once per vertex whose hypercube uses it.
+template <int KIND, class T, int D>
+__device__ __forceinline__ void stage_quantized2(const KParams& P, int64_t v0, double* tile) {
+  for (int e = threadIdx.x; e < Tile3<D>::NSEG * Tile3<D>::SEG; e += BLOCK) {
+    const int seg = e / Tile3<D>::SEG, i = e - seg * Tile3<D>::SEG;
+    int64_t u = v0 + P.loff[seg << 1] + i;
+    u = u < P.nvert ? u : P.nvert - 1;  // such corners lie outside the grid and are masked
+    VtxRef r;
+    if (P.packed) {
+      r.base[0] = r.base[1] = r.base[2] = u;
+    } else {
+      int64_t l[4];
+      decompose<D>(P, u, l);
+      r = make_ref<D>(P, u, l);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      T v = load_val<T>(P, k, r, 0);
+      if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
+      tile[e * 2 + k] = qt(v, P.Q);
+    }
+  }
+}
+
 // Bit i of a sequence of N flags shifted into 32-bit words one at a time
 // (w = (w << 1) | flag, flags 32j.. in word j): the last flag of a word is bit 0.
 template <int N, int I>
@@ -436,13 +463,19 @@ __device__ __forceinline__ T val2(const KParams& P, int k, const Vtx& x, int s) 
   return v;
 }
 
+// Whether stage 1 of a det2 predicate reads its corners from the LDS tile
+// (stage_quantized2): on 2-D spacetime meshes (4 segments, 16 KB); a 3-D one would
+// need 8 segments (33 KB) and runs faster on direct loads.
+template <int D, int M>
+constexpr bool TILE2 = M == 2 && D == 3;
+
 __device__ __forceinline__ double cross2(const double* A, const double* B) {
   // exact for |q| < 2^26: products < 2^52
   return fma(A[0], B[1], -(A[1] * B[0]));
 }
 
 template <int KIND, class T, int D>
-__device__ __forceinline__ Masks classify_tri(const KParams& P, const Vtx& x) {
+__device__ __forceinline__ Masks classify_tri(const KParams& P, const Vtx& x, const double* tile) {
   uint64_t acc = 0, sol = 0, deg = 0;
   constexpr int NC = 1 << D;
   const uint32_t inb = corner_mask<D>(P, x.l);
@@ -451,10 +484,16 @@ __device__ __forceinline__ Masks classify_tri(const KParams& P, const Vtx& x) {
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
     const bool in = (inb >> s) & 1;
+    const double* src = tile + ((s >> 1) * Tile3<D>::SEG + threadIdx.x + (s & 1)) * 2;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const T v = val2<KIND, T, D>(P, k, x, in ? s : 0);
-      const double q = in ? qt(v, P.Q) : 0.0;
+      double q;
+      if constexpr (TILE2<D, 2>) {
+        q = in ? src[k] : 0.0;  // qt(value - threshold) of corner s
+      } else {
+        const T v = val2<KIND, T, D>(P, k, x, in ? s : 0);
+        q = in ? qt(v, P.Q) : 0.0;
+      }
       mx = fmax(mx, fabs(q));
       X[s][k] = q;
     }
@@ -513,7 +552,7 @@ __device__ __forceinline__ Masks classify_fast(const KParams& P, const Vtx& x, c
   if constexpr (M == 3)
     return classify_cp3<T, D>(P, x, tile);
   else if constexpr (M == 2)
-    return classify_tri<KIND, T, D>(P, x);
+    return classify_tri<KIND, T, D>(P, x, reinterpret_cast<const double*>(tile));
   else
     return classify_edge<KIND, T, D>(P, x);
 }
@@ -1015,12 +1054,15 @@ __device__ __forceinline__ void run_items(ItemLds& L, uint64_t w, F&& f) {
 // verbatim sign levels), degm = the verbatim ones among them.
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
-  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : 16;
+  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : (TILE2<D, M> ? Tile3<D>::NSEG * Tile3<D>::SEG * 16 : 16);
   __shared__ __align__(16) float tile[TBYTES / 4];
   const int64_t v0 = blk * BLOCK;
   const int64_t v = v0 + threadIdx.x;
   if constexpr (M == 3) {
     stage_quantized3<T, D>(P, v0, tile);
+    __syncthreads();
+  } else if constexpr (TILE2<D, M>) {
+    stage_quantized2<KIND, T, D>(P, v0, reinterpret_cast<double*>(tile));
     __syncthreads();
   }
   Masks mk{0, 0, 0, 0};
