@@ -18,10 +18,17 @@
 //                                   triple product, d_i = -V_i.
 //   solver                        : test_zero_crossing.cpp / test_3d_cp.cpp answers,
 //                                   numpy solve within 1e-12, exact 0.25 case.
-//   engine end to end             : test_examples.cpp golden counts 52 / 6078 / 66418,
+//   engine end to end             : test_examples.cpp golden counts 6078 / 66418 / 788
+//                                   (the last on a restated woven field),
 //                                   test_engine.cpp structure, test_mesh.cpp counts,
 //                                   brute-force "every m-face of every d-cell" node set,
 //                                   union-find against scipy connected components.
+//                                   The CP3D float case asserted as 52 there
+//                                   (test_examples.cpp:153-186) gives 29 here, as a
+//                                   literal transcription of the reference does.
+// Parity unpinned: d = 3 contour connectivity where a 2-2 split picks its quad
+// diagonal (the reference uses node discovery order, engine.hpp:1335-1337), and the
+// levelset-2D golden 1206, whose input generator is not available.
 #pragma once
 #include <algorithm>
 #include <array>
