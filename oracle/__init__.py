@@ -54,6 +54,13 @@ def lib():
         L.oracle_n_components.restype = C.c_uint64
         L.oracle_n_components.argtypes = [P]
         L.oracle_free.argtypes = [P]
+        L.oracle_lean_run.restype = P
+        L.oracle_lean_run.argtypes = [C.c_int, U64P, U64P, U64P, C.c_int, C.c_int, C.c_int, C.c_double, DP,
+                                      C.c_int, C.POINTER(C.c_void_p), I64P, C.c_int]
+        L.oracle_lean_sizes.argtypes = [P, U64P, U64P, U64P]
+        L.oracle_lean_nodes.argtypes = [P, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_lean_edges.argtypes = [P, C.c_uint64, C.c_uint64, C.c_void_p]
+        L.oracle_lean_free.argtypes = [P]
         L.oracle_sign.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_double]
         L.oracle_det2.argtypes = [C.c_int, DP, DP, C.c_uint64, C.c_uint64, C.c_double]
         L.oracle_det3.argtypes = [C.c_int, DP, DP, DP, C.c_double]
@@ -140,6 +147,63 @@ def run(comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e
                          st.ctypes.data_as(C.POINTER(C.c_int64)), sptr,
                          sst.ctypes.data_as(C.POINTER(C.c_int64)), nthreads, C.c_uint64(time_stride))
     return _collect(h)
+
+
+class Lean:
+    """Full-size mode (lean.hpp) for star-edge predicates: the same FeatureComplex as
+    run(), held compactly; read it in ranges with nodes() / edges()."""
+
+    def __init__(self, comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e6, nthreads=0):
+        comps = list(comps)
+        dtype = 0 if comps[0].dtype == np.float32 else 1
+        d = comps[0].ndim
+        dims = comps[0].shape if dims is None else dims
+        item = comps[0].itemsize
+        ptrs = (C.c_void_p * len(comps))(*[c.ctypes.data for c in comps])
+        st = np.zeros(len(comps) * 4, np.int64)
+        for i, c in enumerate(comps):
+            assert c.dtype == comps[0].dtype
+            for j in range(d):
+                st[i * 4 + j] = c.strides[j] // item
+        dims_a, dims_p = _u64(list(dims) + [1] * (4 - d))
+        off_a, off_p = _u64((list(offset) if offset is not None else [0] * d) + [0] * (4 - d))
+        g_a, g_p = _u64((list(gdims) if gdims is not None else list(dims)) + [1] * (4 - d))
+        thr_a, thr_p = _dp(list(thr) + [0.0] * (2 - len(thr)))
+        L = lib()
+        self._h = L.oracle_lean_run(d, dims_p, off_p, g_p, kind, m, dtype, Q, thr_p, len(comps), ptrs,
+                                    st.ctypes.data_as(C.POINTER(C.c_int64)), nthreads)
+        if not self._h:
+            raise RuntimeError("oracle failed: " + L.oracle_error().decode())
+        self.m = m
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        L.oracle_lean_sizes(self._h, C.byref(a), C.byref(b), C.byref(c))
+        self.n_nodes, self.n_edges, self.n_components = a.value, b.value, c.value
+
+    def nodes(self, begin=0, count=None):
+        count = self.n_nodes - begin if count is None else count
+        nv = self.m + 1
+        verts = np.zeros((count, nv), np.uint64)
+        lam = np.zeros((count, nv), np.float64)
+        track = np.zeros(count, np.uint64)
+        lib().oracle_lean_nodes(self._h, begin, count, verts.ctypes.data, lam.ctypes.data, track.ctypes.data)
+        return verts, lam, track
+
+    def edges(self, begin=0, count=None):
+        count = self.n_edges - begin if count is None else count
+        out = np.zeros((count, 2), np.uint32)
+        lib().oracle_lean_edges(self._h, begin, count, out.ctypes.data)
+        return out
+
+    def close(self):
+        if self._h:
+            lib().oracle_lean_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def extruded_time_stride(spatial_global_dims):

@@ -5,6 +5,7 @@
 #include <memory>
 #include <string>
 
+#include "lean.hpp"
 #include "oracle.hpp"
 
 using namespace oracle;
@@ -110,6 +111,59 @@ void* oracle_run_stream(int sd, const uint64_t* spatial, int nt, int kind, int m
     return nullptr;
   }
 }
+
+// Full-size mode (lean.hpp): same arguments as oracle_run, star-edge predicates only.
+void* oracle_lean_run(int d, const uint64_t* dims, const uint64_t* off, const uint64_t* gdims, int kind, int m,
+                      int dtype, double Q, const double* thr, int ncomp, const void* const* ptrs,
+                      const int64_t* strides, int nthreads) {
+  try {
+    Mesh mesh;
+    mesh.d = d;
+    for (int j = 0; j < d; ++j) {
+      mesh.dims[j] = dims[j];
+      mesh.off[j] = off ? off[j] : 0;
+      mesh.gdims[j] = gdims ? gdims[j] : dims[j];
+    }
+    auto L = std::make_unique<Lean>();
+    if (dtype == 0)
+      execute_lean(make_params<float>(kind, m, Q, thr, ncomp, ptrs, strides, nullptr, nullptr), mesh, nthreads, *L);
+    else
+      execute_lean(make_params<double>(kind, m, Q, thr, ncomp, ptrs, strides, nullptr, nullptr), mesh, nthreads, *L);
+    return L.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void oracle_lean_sizes(void* hp, uint64_t* n_nodes, uint64_t* n_edges, uint64_t* n_components) {
+  const auto* L = static_cast<const Lean*>(hp);
+  *n_nodes = L->keys.size();
+  *n_edges = L->edges.size();
+  *n_components = L->n_components;
+}
+
+// Nodes [begin, begin + count): sorted vertex ids, lambda, track ids (any pointer may be NULL).
+void oracle_lean_nodes(void* hp, uint64_t begin, uint64_t count, uint64_t* verts, double* lam, uint64_t* track) {
+  const auto* L = static_cast<const Lean*>(hp);
+  const int nv = L->m + 1;
+  for (uint64_t i = 0; i < count; ++i) {
+    if (verts) L->vertices(begin + i, verts + i * nv);
+    for (int k = 0; k < nv && lam; ++k) lam[i * nv + k] = L->lambda[(begin + i) * nv + k];
+    if (track) track[i] = L->track[begin + i];
+  }
+}
+
+// Edges [begin, begin + count) as (a, b) uint32 pairs.
+void oracle_lean_edges(void* hp, uint64_t begin, uint64_t count, uint32_t* out) {
+  const auto* L = static_cast<const Lean*>(hp);
+  for (uint64_t i = 0; i < count; ++i) {
+    out[2 * i] = (uint32_t)(L->edges[begin + i] >> 32);
+    out[2 * i + 1] = (uint32_t)L->edges[begin + i];
+  }
+}
+
+void oracle_lean_free(void* hp) { delete static_cast<Lean*>(hp); }
 
 uint64_t oracle_n_nodes(void* hp) { return static_cast<Handle*>(hp)->cx.vertices.size(); }
 int oracle_node_dim(void* hp) { return static_cast<Handle*>(hp)->m; }
