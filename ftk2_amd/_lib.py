@@ -58,6 +58,11 @@ class Job(C.Structure):
     _fields_ = [("mesh", MeshDesc), ("comps", C.POINTER(FieldView)), ("scalar", C.POINTER(FieldView))]
 
 
+class OutputBuffers(C.Structure):
+    _fields_ = [("keys", C.c_void_p), ("lambda_", C.c_void_p), ("track_id", C.c_void_p), ("scalar", C.c_void_p),
+                ("cells", C.c_void_p), ("node_capacity", C.c_uint64), ("cell_capacity", C.c_uint64)]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("grid_per_cu", C.c_int), ("chunk", C.c_int), ("use_graph", C.c_int)]
 
@@ -110,6 +115,10 @@ SIGNATURES = {
     "ftk2h_server_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
     "ftk2h_server_destroy": (None, [_P]),
+    "ftk2h_extract_regular_noalloc": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(PredicateDesc),
+                                                C.POINTER(FieldView), C.c_int, C.POINTER(FieldView),
+                                                C.POINTER(OutputBuffers), C.POINTER(C.c_uint64),
+                                                C.POINTER(C.c_uint64), _P]),
 }
 
 _lib = None

@@ -66,7 +66,84 @@ RunFn find_entry(int kind, int d, int m, int dtype) {
   return nullptr;
 }
 
-}  // namespace
+// Request checks shared by the extraction entry points: predicate, dtype, component
+// count, a built (kind, d, m, dtype) pipeline and the mesh descriptor.
+int check_request(const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred, const ftk2h_field_view* comps,
+                  int ncomp, RunFn* fn) {
+  const int d = mesh->ndims;
+  if (d < 2 || d > 4) return fail(FTK2H_E_UNSUPPORTED, "ndims must be 2..4");
+  const int m = pred->codimension;
+  const int need = components_needed(pred);
+  if (need < 0) return FTK2H_E_INVALID;
+  if (pred->dtype != FTK2H_F32 && pred->dtype != FTK2H_F64) return fail(FTK2H_E_INVALID, "bad dtype");
+  if (ncomp != need || !comps) return fail(FTK2H_E_INVALID, "wrong number of field components");
+  *fn = find_entry(pred->kind, d, m, pred->dtype);
+  if (!*fn)
+    return fail(FTK2H_E_UNSUPPORTED, "predicate kind " + std::to_string(pred->kind) + " with d=" +
+                                         std::to_string(d) + " m=" + std::to_string(m) + " not built");
+  uint64_t nvert = 0;
+  bool has_cells = false;
+  if (int rc = check_mesh(mesh, d, &nvert, &has_cells)) return rc;
+  for (int c = 0; c < ncomp; ++c)
+    if (!comps[c].ptr) return fail(FTK2H_E_INVALID, "null field pointer");
+  return FTK2H_OK;
+}
+
+void init_result(ftk2h_result* res, const ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh,
+                 const ftk2h_predicate_desc* pred) {
+  res->device = ctx->device;
+  res->kind = pred->kind;
+  res->d = mesh->ndims;
+  res->m = pred->codimension;
+  res->dtype = pred->dtype;
+  for (int j = 0; j < mesh->ndims; ++j) res->gdims[j] = mesh->global_dims[j];
+  res->tstride = mesh->time_id_stride;
+  // ContourPredicate::extract_it stores its threshold as the scalar (predicate.hpp:150)
+  if (pred->kind == FTK2H_PRED_CONTOUR) res->scalar_value = (float)pred->thresholds[0];
+}
+
+// Mesh, predicate and field fields of P.  Device views are read in place; host views
+// (allow_host) are packed into a contiguous (x fastest) device copy held by the context.
+int bind_request(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred,
+                 const ftk2h_field_view* comps, int ncomp, const ftk2h_field_view* scalar, bool allow_host,
+                 hipStream_t st, KParams& P, uint64_t* nvert_out, bool* has_cells) {
+  const int d = mesh->ndims;
+  uint64_t nvert = 0;
+  if (int rc = check_mesh(mesh, d, &nvert, has_cells)) return rc;
+  *nvert_out = nvert;
+  if (!*has_cells) return FTK2H_OK;
+  init_params(P, mesh, pred, nvert);
+  const size_t esz = pred->dtype == FTK2H_F64 ? 8 : 4;
+  auto bind = [&](const ftk2h_field_view& fv, int slot, const void** ptr, int64_t* strides) -> int {
+    if (fv.on_device) {
+      *ptr = fv.ptr;
+      for (int j = 0; j < 4; ++j) strides[j] = j < d ? fv.strides[j] : 0;
+      return FTK2H_OK;
+    }
+    if (!allow_host) return fail(FTK2H_E_INVALID, "this entry point reads device views only (on_device = 1)");
+    std::vector<char> pack(nvert * esz);
+    pack_host_view(fv, mesh, esz, nvert, pack.data());
+    int rc = ctx->fields[slot].ensure(nvert * esz);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->fields[slot].p, pack.data(), nvert * esz, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *ptr = ctx->fields[slot].p;
+    packed_strides(mesh, strides);
+    return FTK2H_OK;
+  };
+  for (int c = 0; c < ncomp; ++c) {
+    int rc = bind(comps[c], c, &P.comp[c], P.cst[c]);
+    if (rc) return rc;
+  }
+  if (scalar && scalar->ptr && pred->kind == FTK2H_PRED_CRITICAL_POINT) {
+    int rc = bind(*scalar, 3, &P.scal, P.sst);
+    if (rc) return rc;
+  }
+  finish_params(P, mesh, ncomp);
+  return FTK2H_OK;
+}
+
+}  // namespace ftk2h_internal
 
 namespace {
 // Vertex ids of the node with key k: smallest vertex + chain offsets.
@@ -121,6 +198,7 @@ int ftk2h_ctx_create(int device, ftk2h_ctx** out) {
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   ctx->pool = std::make_shared<DevPool>();
   ctx->pool->device = device;
+  if (int rc = noalloc_init(device)) return rc;  // fixed scratch array of ftk2h_extract_regular_noalloc
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b));
   ctx->pool->cap = total_b / 8;  // idle result blocks kept for reuse
@@ -157,72 +235,22 @@ int ftk2h_extract_regular(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk
                           const ftk2h_options* opt, ftk2h_result** out, void* hip_stream) {
   if (!ctx || !mesh || !pred || !out) return fail(FTK2H_E_INVALID, "null argument");
   *out = nullptr;
-  const int d = mesh->ndims;
-  if (d < 2 || d > 4) return fail(FTK2H_E_UNSUPPORTED, "ndims must be 2..4");
-  const int m = pred->codimension;
-  const int need = components_needed(pred);
-  if (need < 0) return FTK2H_E_INVALID;
-  if (pred->dtype != FTK2H_F32 && pred->dtype != FTK2H_F64) return fail(FTK2H_E_INVALID, "bad dtype");
-  if (ncomp != need || !comps) return fail(FTK2H_E_INVALID, "wrong number of field components");
-  RunFn fn = find_entry(pred->kind, d, m, pred->dtype);
-  if (!fn)
-    return fail(FTK2H_E_UNSUPPORTED, "predicate kind " + std::to_string(pred->kind) + " with d=" +
-                                         std::to_string(d) + " m=" + std::to_string(m) + " not built");
-  uint64_t nvert = 0;
-  bool has_cells = false;
-  if (int rc = check_mesh(mesh, d, &nvert, &has_cells)) return rc;
-  for (int c = 0; c < ncomp; ++c)
-    if (!comps[c].ptr) return fail(FTK2H_E_INVALID, "null field pointer");
-
+  RunFn fn = nullptr;
+  if (int rc = check_request(mesh, pred, comps, ncomp, &fn)) return rc;
   std::lock_guard<std::mutex> guard(ctx->lock);
   HIP_TRY(hipSetDevice(ctx->device));
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
 
   auto res = std::make_unique<ftk2h_result>();
-  res->device = ctx->device;
-  res->kind = pred->kind;
-  res->d = d;
-  res->m = m;
-  res->dtype = pred->dtype;
-  for (int j = 0; j < d; ++j) res->gdims[j] = mesh->global_dims[j];
-  res->tstride = mesh->time_id_stride;
-  // ContourPredicate::extract_it stores its threshold as the scalar (predicate.hpp:150)
-  if (pred->kind == FTK2H_PRED_CONTOUR) res->scalar_value = (float)pred->thresholds[0];
+  init_result(res.get(), ctx, mesh, pred);
+  KParams P;
+  uint64_t nvert = 0;
+  bool has_cells = false;
+  if (int rc = bind_request(ctx, mesh, pred, comps, ncomp, scalar, true, st, P, &nvert, &has_cells)) return rc;
   if (!has_cells) {  // no hypercube: the reference visits nothing
     *out = res.release();
     return FTK2H_OK;
   }
-
-  KParams P;
-  init_params(P, mesh, pred, nvert);
-  const size_t esz = pred->dtype == FTK2H_F64 ? 8 : 4;
-  // Field views: device views are read in place; host views are packed into a
-  // contiguous (x fastest) device copy.
-  auto bind = [&](const ftk2h_field_view& fv, int slot, const void** ptr, int64_t* strides) -> int {
-    if (fv.on_device) {
-      *ptr = fv.ptr;
-      for (int j = 0; j < 4; ++j) strides[j] = j < d ? fv.strides[j] : 0;
-      return FTK2H_OK;
-    }
-    std::vector<char> pack(nvert * esz);
-    pack_host_view(fv, mesh, esz, nvert, pack.data());
-    int rc = ctx->fields[slot].ensure(nvert * esz);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(ctx->fields[slot].p, pack.data(), nvert * esz, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    *ptr = ctx->fields[slot].p;
-    packed_strides(mesh, strides);
-    return FTK2H_OK;
-  };
-  for (int c = 0; c < ncomp; ++c) {
-    int rc = bind(comps[c], c, &P.comp[c], P.cst[c]);
-    if (rc) return rc;
-  }
-  if (scalar && scalar->ptr && pred->kind == FTK2H_PRED_CRITICAL_POINT) {
-    int rc = bind(*scalar, 3, &P.scal, P.sst);
-    if (rc) return rc;
-  }
-  finish_params(P, mesh, ncomp);
   const int64_t nb = (int64_t)((nvert + BLOCK - 1) / BLOCK);
   int rc;
   if ((rc = ctx->mask.ensure(nvert * 8))) return rc;

@@ -198,6 +198,21 @@ struct ftk2h_result {
 
 namespace ftk2h_internal {
 
+// Output arrays supplied by the caller and work memory that must not be allocated
+// (ftk2h_extract_regular_noalloc): the pipeline writes the result there and fails
+// with FTK2H_E_OVERFLOW when a capacity is too small.
+struct FixedOut {
+  uint64_t* keys;
+  void* lambda;
+  uint32_t* track;
+  float* scalar;
+  uint32_t* cells;
+  uint64_t node_cap, cell_cap;  // caller's capacities (nodes, cells)
+  char* tail;                   // scratch for the union-find parents and the cross-workgroup
+  uint64_t tail_bytes;          // edges, split once the node and cell counts are known
+  uint32_t* scan_sums;
+};
+
 struct RunArgs {
   ftk2h_ctx* ctx;
   hipStream_t st;
@@ -206,9 +221,22 @@ struct RunArgs {
   uint64_t node_cap, cell_cap;
   bool sync;
   ftk2h_result* res;
+  const FixedOut* fixed = nullptr;  // null: result arrays from the context's pool
 };
 
 using RunFn = int (*)(RunArgs&);
+
+// Entry-point helpers (ftk2h.hip).
+int check_request(const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred, const ftk2h_field_view* comps,
+                  int ncomp, RunFn* fn);
+void init_result(ftk2h_result* res, const ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh,
+                 const ftk2h_predicate_desc* pred);
+int bind_request(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred,
+                 const ftk2h_field_view* comps, int ncomp, const ftk2h_field_view* scalar, bool allow_host,
+                 hipStream_t st, KParams& P, uint64_t* nvert, bool* has_cells);
+// Device address of the fixed scratch array on the current device (noalloc.hip); called
+// once per device when a context is created.
+int noalloc_init(int device);
 
 // ---- request checks and kernel parameters shared by every entry point ----------
 
@@ -403,6 +431,7 @@ int run_pipeline(RunArgs& A) {
   hipStream_t st = A.st;
   const int64_t nb = A.nb;
   auto rec = [&](int i) { return hipEventRecord(ctx->ev[i], st); };
+  uint32_t* scan_sums = A.fixed ? A.fixed->scan_sums : static_cast<uint32_t*>(ctx->scan_sums.p);
   HIP_TRY(rec(0));
   if (nb > 0) {
     k_classify<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
@@ -410,13 +439,13 @@ int run_pipeline(RunArgs& A) {
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(1));
-  HIP_TRY(scan_u32(P.bsum, P.bsum, (uint64_t)nb, static_cast<uint32_t*>(ctx->scan_sums.p), st));
+  HIP_TRY(scan_u32(P.bsum, P.bsum, (uint64_t)nb, scan_sums, st));
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(2));
   if (nb > 0) k_offsets<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(3));
-  HIP_TRY(scan_u32(P.ebsum, P.ebsum, (uint64_t)nb, static_cast<uint32_t*>(ctx->scan_sums.p), st));
+  HIP_TRY(scan_u32(P.ebsum, P.ebsum, (uint64_t)nb, scan_sums, st));
   HIP_TRY(hipGetLastError());
   uint32_t totals[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(&totals[0], P.bsum + nb, 4, hipMemcpyDeviceToHost, st));
@@ -434,6 +463,32 @@ int run_pipeline(RunArgs& A) {
   R->n_nodes = nn;
   R->ntypes = Chains<D, M>::N;
   uint32_t* parent = nullptr;
+  if (const FixedOut* F = A.fixed) {
+    if (nn > F->node_cap || (NV > 0 && nc > F->cell_cap)) {
+      R->conn_cells[0] = nc;  // required sizes for the caller
+      return fail(FTK2H_E_OVERFLOW, "output capacity: " + std::to_string(nn) + " nodes and " +
+                                        std::to_string(nc) + " cells required");
+    }
+    const uint64_t pbytes = (nn * 4 + 255) & ~(uint64_t)255;
+    const uint64_t xbytes = NV > 0 ? nc * (uint64_t)(NV > 1 ? NV - 1 : 1) * 8 : 0;
+    if (pbytes + xbytes > F->tail_bytes)
+      return fail(FTK2H_E_OVERFLOW, "scratch capacity: " + std::to_string(nn) + " nodes and " + std::to_string(nc) +
+                                        " cells need " + std::to_string(pbytes + xbytes) + " bytes, " +
+                                        std::to_string(F->tail_bytes) + " left in the fixed scratch array");
+    if (P.scal && !F->scalar) return fail(FTK2H_E_INVALID, "a scalar field needs a scalar output array");
+    R->keys = F->keys;
+    R->lambda = F->lambda;
+    R->track = F->track;
+    R->scalar = P.scal ? F->scalar : nullptr;
+    parent = reinterpret_cast<uint32_t*>(F->tail);
+    if (NV > 0 && nc) {
+      P.xedges = reinterpret_cast<uint32_t*>(F->tail + pbytes);
+      R->n_conn = 1;
+      R->conn_dim[0] = NV - 1;
+      R->conn_cells[0] = nc;
+      R->conn[0] = F->cells;
+    }
+  } else {
   R->pool = ctx->pool;
   DevPool& pool = *ctx->pool;
   if (nn) {
@@ -452,6 +507,7 @@ int run_pipeline(RunArgs& A) {
     R->conn_dim[0] = NV - 1;
     R->conn_cells[0] = nc;
     HIP_TRY(pool.get(reinterpret_cast<void**>(&R->conn[0]), nc * NV * 4));
+  }
   }
   P.keys = R->keys;
   P.lambda = R->lambda;

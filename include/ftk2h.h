@@ -268,6 +268,36 @@ int ftk2h_server_stats(const ftk2h_server* s, uint64_t* reserved_bytes, uint64_t
                        int* queued_timesteps);
 void ftk2h_server_destroy(ftk2h_server* s);
 
+/* ---- Extraction without device allocation ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same extraction as ftk2h_extract_regular, with the result written
+ * into the caller's device arrays and every work array (per-vertex masks and offsets,
+ * tile counts, union-find parents, cross-workgroup edges) taken from one fixed
+ * device array of FTK2H_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from
+ * the first ftk2h_ctx_create on that GPU until the process ends.  Calls no hipMalloc
+ * or hipFree.  Field views must be device views (on_device = 1).  Not built for
+ * star-fan cells (codimension 2 on 3-D + time meshes: FTK2H_E_UNSUPPORTED).
+ * Every call on one GPU uses that GPU's scratch array: calls that overlap on one GPU
+ * (other host threads, other contexts or streams) race and must be serialized by
+ * the caller.  FTK2H_E_OVERFLOW when the mesh's work arrays exceed the scratch array
+ * or the counts exceed the capacities below (*n_nodes and *n_cells then hold the
+ * required counts when they are known). */
+#define FTK2H_NOALLOC_SCRATCH_BYTES (384ull << 20)
+typedef struct {
+  uint64_t* keys;         /* node_capacity node keys (v_min * n_types + type), sorted */
+  void* lambda;           /* node_capacity * (m+1) values of the predicate dtype */
+  uint32_t* track_id;     /* node_capacity: minimum canonical index of the component */
+  float* scalar;          /* node_capacity, required when a scalar field is given, else NULL */
+  uint32_t* cells;        /* cell_capacity * (cell dim + 1) canonical indices, sorted */
+  uint64_t node_capacity;
+  uint64_t cell_capacity;
+} ftk2h_output_buffers;
+int ftk2h_extract_regular_noalloc(ftk2h_ctx* ctx, const ftk2h_mesh_desc* mesh, const ftk2h_predicate_desc* pred,
+                                  const ftk2h_field_view* comps, int ncomp, const ftk2h_field_view* scalar,
+                                  const ftk2h_output_buffers* out, uint64_t* n_nodes, uint64_t* n_cells,
+                                  void* hip_stream);
+
 const char* ftk2h_last_error(void);
 
 #ifdef __cplusplus
