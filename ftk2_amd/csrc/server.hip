@@ -4,7 +4,8 @@
 // union-find frontier stay on the device).  A step takes every admitted stream
 // that has a queued timestep, forms its next slab, and extracts the slabs of one
 // predicate template in one batched pass (ftk2h_batch_run), except slabs large
-// enough to fill the device, which run one by one; streams join at the
+// enough to fill the device and slabs of templates that are not batched (star-fan
+// cells), which run one by one; streams join at the
 // first step after admission and leave when closed and drained.  Because a slab's
 // extraction is independent of the other jobs in its batch, each result equals
 // the one the stream gets alone.
@@ -66,6 +67,11 @@ struct ftk2h_server {
 namespace {
 
 int n_types_of(int d, int m) { return n_types_rt(d, m); }
+
+// Whether the batched pass is built for a predicate template: star-fan cells
+// (codimension 2 on 3-D + time meshes) need a global sort and are not batched, so
+// their slabs run through the single-mesh pipeline.
+bool batchable(const ftk2h_predicate_desc& p, int d) { return !(d == 4 && p.codimension == 2); }
 
 // Device bytes reserved for a stream: its two resident timesteps, the per-slab
 // work arrays of a batched pass (u64 mask + u32 offset per slab vertex) and an
@@ -218,7 +224,7 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
       stream_slab_desc(S.s, &p.job.mesh, p.views);
       uint64_t nv = 1;
       for (int j = 0; j < p.job.mesh.ndims; ++j) nv *= p.job.mesh.local_dims[j];
-      if (nv >= SOLO_SLAB_VERTICES)
+      if (nv >= SOLO_SLAB_VERTICES || !batchable(S.pred, S.d))
         solo.push_back(p);
       else
         groups[key_of(S.pred, S.d)].push_back(p);

@@ -141,6 +141,11 @@ struct ftk2h_stream {
   std::vector<uint32_t> h_cells;
   uint32_t* scratch = nullptr;  // small device words (lower bound, bad-key count)
   float ms_total = 0.f;
+  // The last committed slab: the re-indexing kernels queued on st read its arrays, so
+  // they go back to the context's pool only after retiring_ev has completed.
+  ftk2h_result* retiring = nullptr;
+  hipEvent_t retiring_ev = nullptr;
+  uint64_t node_limit = 0xFFFFFFFFull;  // committed indices are uint32 (below this)
 };
 
 namespace {
@@ -153,11 +158,30 @@ int slab_lower_count(ftk2h_stream* S, const ftk2h_result* R, uint64_t slab_t, ui
   return FTK2H_OK;
 }
 
+// Frees the result that retiring_ev guards (waiting for it first) and, when R is
+// given, makes R the guarded one: an event recorded on st after the kernels that read it.
+int retire_result(ftk2h_stream* S, ftk2h_result* R) {
+  if (S->retiring) {
+    HIP_TRY(hipEventSynchronize(S->retiring_ev));
+    ftk2h_result_free(S->retiring);
+    S->retiring = nullptr;
+  }
+  if (R) {
+    S->retiring = R;
+    HIP_TRY(hipEventRecord(S->retiring_ev, S->st));
+  }
+  return FTK2H_OK;
+}
+
 // Commit the open slab: its first `nlow` nodes (or all of them when `next` is null)
 // and its cells re-indexed against `next`.
 int commit_open(ftk2h_stream* S, const ftk2h_result* next, uint32_t next_nlow) {
   ftk2h_result* R = S->open;
   const uint32_t nk = next ? S->open_nlow : (uint32_t)R->n_nodes;
+  // committed indices, the next slab's lower plane included, are uint32 node indices
+  if ((uint64_t)S->committed + nk + (next ? next_nlow : 0) >= S->node_limit)
+    return fail(FTK2H_E_OVERFLOW, "stream: more than " + std::to_string(S->node_limit - 1) +
+                                      " committed nodes (uint32 indices)");
   const size_t lsz = (size_t)(R->m + 1) * S->esz;
   int rc;
   const uint64_t dev_nodes = (uint64_t)S->committed - S->fr.F;  // committed nodes still on the device
@@ -348,8 +372,9 @@ int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow) {
   int rc;
   if (S->open) {
     rc = commit_open(S, R, nlow);
-    ftk2h_result_free(S->open);
+    const int rc2 = retire_result(S, S->open);
     S->open = nullptr;
+    if (!rc) rc = rc2;
     if (!rc && S->keys.bytes + S->lambda.bytes + S->cells.bytes > S->retire_bytes) rc = retire(S);
     if (rc) {
       ftk2h_result_free(R);
@@ -403,6 +428,12 @@ int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
   for (int c = 0; c < ncomp; ++c) HIP_TRY(hipMalloc(&S->fields[c], 2 * S->nsp * S->esz));
+  HIP_TRY(hipEventCreateWithFlags(&S->retiring_ev, hipEventDisableTiming));
+  // FTK2H_STREAM_NODE_LIMIT lowers the committed-node limit (tests drive the overflow check)
+  if (const char* lim = std::getenv("FTK2H_STREAM_NODE_LIMIT")) {
+    const unsigned long long v = std::strtoull(lim, nullptr, 10);
+    if (v > 0 && v < S->node_limit) S->node_limit = v;
+  }
   HIP_TRY(hipMalloc(&S->scratch, 64));
   HIP_TRY(hipMemset(S->scratch, 0, 64));  // [1] counts cells whose upper node was not found
   *out = S.release();
@@ -445,9 +476,10 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
   int rc;
   if (S->open) {
     rc = commit_open(S, nullptr, 0);
-    ftk2h_result_free(S->open);
+    const int rc2 = retire_result(S, S->open);
     S->open = nullptr;
     if (rc) return rc;
+    if (rc2) return rc2;
   }
   uint32_t bad = 0;
   HIP_TRY(hipMemcpyAsync(&bad, S->scratch + 1, 4, hipMemcpyDeviceToHost, S->st));
@@ -510,6 +542,7 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
     }
   }
   HIP_TRY(hipStreamSynchronize(S->st));
+  if ((rc = retire_result(S, nullptr))) return rc;
   res->phase_ms[0] = S->ms_total;
   *out = res.release();
   return FTK2H_OK;
@@ -520,6 +553,8 @@ void ftk2h_stream_close(ftk2h_stream* S) {
   (void)hipSetDevice(S->ctx->device);
   if (S->st) (void)hipStreamSynchronize(S->st);
   if (S->open) ftk2h_result_free(S->open);
+  if (S->retiring) ftk2h_result_free(S->retiring);
+  if (S->retiring_ev) (void)hipEventDestroy(S->retiring_ev);
   for (auto* f : S->fields)
     if (f) (void)hipFree(f);
   if (S->scratch) (void)hipFree(S->scratch);

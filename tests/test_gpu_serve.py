@@ -61,7 +61,7 @@ def mixed_requests(n, seed, device=False, arrival_rate=None):
 
 
 def alone(rq):
-    st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]))
+    st = engine.Stream(rq.spatial, rq.kind, rq.m, rq.dtype, len(rq.timesteps[0]), thresholds=rq.thresholds)
     for comps in rq.timesteps:
         st.append(comps)
     res = st.finish().to_complex()
@@ -161,3 +161,69 @@ def test_device_filling_slabs_run_alone_in_the_mix():
             assert len(got.vertices) > 100
             one = engine.extract(engine.RegularSimplicialMesh((1100, 1000, 3)), 0, 2, list(big)).to_complex()
             same(got, one, "big request vs one-shot")
+
+
+def scalar_request(rid, kind, spatial, T, seed, dtype=np.float32, thr=(0.0, 0.0)):
+    """A contour (one field) or fiber (two fields) stream request on a Gaussian sum
+    minus its mean."""
+    nf = 1 if kind == _lib.PRED_CONTOUR else 2
+    fields = [synth.gaussians(spatial, T, 8, seed=seed + 17 * k, gradient=False, subtract_mean=True, dtype=dtype)[0]
+              for k in range(nf)]
+    m = 1 if kind == _lib.PRED_CONTOUR else 2
+    dt = _lib.F32 if dtype == np.float32 else _lib.F64
+    hc = int(np.prod([s - 1 for s in spatial])) * max(0, T - 1)
+    return serve.Request(rid, 0.0, tuple(spatial), kind, m, dt, [[f[..., t] for f in fields] for t in range(T)],
+                         thresholds=thr, simplices=hc, meta={"comps": fields})
+
+
+def test_contour_and_fiber_requests_in_the_mix():
+    """Contour and fiber streams (fiber on 3-D + time has star-fan cells, which are
+    not batched and take the single-mesh pipeline) served beside critical-point
+    streams: every result equals the request alone and its one-shot extraction."""
+    reqs = mixed_requests(5, seed=6)
+    n0 = len(reqs)
+    reqs.append(scalar_request(n0, _lib.PRED_FIBER, (9, 8, 10), 4, 61, thr=(0.01, -0.02)))
+    reqs.append(scalar_request(n0 + 1, _lib.PRED_FIBER, (24, 20), 5, 62))
+    reqs.append(scalar_request(n0 + 2, _lib.PRED_CONTOUR, (12, 11, 10), 4, 63))
+    reqs.append(scalar_request(n0 + 3, _lib.PRED_CONTOUR, (30, 26), 3, 64, dtype=np.float64, thr=(0.02, 0.0)))
+    srv = engine.Server(64 << 30)
+    outs, wall, rej = serve.run(srv, reqs, keep_results=True)
+    assert rej == [] and len(outs) == len(reqs)
+    for o in outs:
+        rq = reqs[o.rid]
+        say(f"[serve] request {o.rid}: kind={rq.kind} spatial={rq.spatial} T={len(rq.timesteps)} m={rq.m}")
+        got = o.result.to_complex()
+        same(got, alone(rq), f"request {o.rid}")
+        if o.rid >= n0:
+            comps = rq.meta["comps"]
+            one = engine.extract(engine.RegularSimplicialMesh(list(comps[0].shape)), rq.kind, rq.m, comps,
+                                 thresholds=rq.thresholds).to_complex()
+            assert len(one.vertices) > 0
+            same(got, one, f"request {o.rid} vs one-shot")
+
+
+def test_several_device_filling_slabs_in_one_step():
+    """Two or more single-mesh slabs per step: each slab's committed cells are
+    re-indexed on its stream after the next slab's extraction, and the pool blocks of
+    the slab they read are reused only once that has run.  Every result equals its
+    one-shot extraction."""
+    big = [synth.gaussians((1100, 1000), 4, 12, seed=70 + k) for k in range(2)]
+    cube = synth.cp3d_config(n=(132, 128, 128), nt=3, ngauss=8, seed=72)
+    reqs = [serve.Request(k, 0.0, (1100, 1000), _lib.PRED_CRITICAL_POINT, 2, _lib.F64,
+                          [[c[..., t] for c in big[k]] for t in range(4)], simplices=1099 * 999 * 3 * 12,
+                          meta={"comps": big[k]}) for k in range(2)]
+    reqs.append(serve.Request(2, 0.0, (132, 128, 128), _lib.PRED_CRITICAL_POINT, 3, _lib.F64,
+                              [[c[..., t] for c in cube] for t in range(3)], simplices=131 * 127 * 127 * 2 * 60,
+                              meta={"comps": cube}))
+    srv = engine.Server(64 << 30)
+    outs, wall, rej = serve.run(srv, reqs, keep_results=True)
+    assert rej == [] and len(outs) == 3
+    for o in outs:
+        rq = reqs[o.rid]
+        comps = rq.meta["comps"]
+        one = engine.extract(engine.RegularSimplicialMesh(list(comps[0].shape)), rq.kind, rq.m,
+                             list(comps)).to_complex()
+        got = o.result.to_complex()
+        say(f"[serve] request {o.rid}: {len(got.vertices)} nodes")
+        assert len(got.vertices) > 100
+        same(got, one, f"request {o.rid} vs one-shot")

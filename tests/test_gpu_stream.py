@@ -30,20 +30,24 @@ def same(a, b):
         assert np.array_equal(a.connectivity[d], b.connectivity[d])
 
 
-FIBER_SKIP = pytest.mark.skip(
-    reason="one GPU run ended with a host segmentation fault inside the oracle's execute_stream "
-    "restatement for this case, after the stream had matched the one-shot extraction; the cause is "
-    "not found (the same oracle call is AddressSanitizer-clean on the CPU), so the case stays off "
-    "the GPU.  Fiber streams stay covered by test_stream_retirement_equals_oneshot[fiber-*].")
+# One GPU test run ended with a host segmentation fault inside the oracle's
+# execute_stream restatement during the fiber case, after the GPU stream had matched
+# the one-shot extraction.  It did not recur on the CPU (torch imported first, so the
+# oracle runs on torch's bundled libgomp; 1, 16, 64 threads; AddressSanitizer), and
+# its cause is not known, so the oracle's stream restatement is not run on the GPU
+# host for fiber: the fiber stream is compared with the oracle's one-shot
+# restatement instead, and tests/test_oracle_engine.py checks on the CPU that the
+# stream restatement equals the one-shot one for every predicate.
+ORACLE_STREAM_ON_GPU_HOST = {"cp3d", "cp2d", "contour", "cp3d_float"}
 
 
-@pytest.mark.parametrize("case", ["cp3d", "cp2d", "contour", pytest.param("fiber", marks=FIBER_SKIP), "cp3d_float"])
+@pytest.mark.parametrize("case", ["cp3d", "cp2d", "contour", "fiber", "cp3d_float"])
 def test_stream_equals_oneshot(case):
     print(f"[stream] {case}", flush=True)
     if case == "cp3d":
         comps, kind, m, thr = synth.cp3d_config(n=18, nt=6, seed=3), 0, 3, (0, 0)
     elif case == "cp3d_float":
-        comps = synth.gaussians((16, 14, 12), 5, 6, sigma=(8, 24), seed=4, noise=1e-3, zero_frac=0.02,
+        comps = synth.gaussians((14, 13, 12), 5, 6, sigma=(8, 24), seed=4, noise=0.02, zero_frac=0.02,
                                 dup_frac=0.02, dtype=np.float32)
         kind, m, thr = 0, 3, (0, 0)
     elif case == "cp2d":
@@ -58,9 +62,13 @@ def test_stream_equals_oneshot(case):
     one = engine.extract(mesh, kind, m, comps, thr).to_complex()
     st = run_stream(comps, kind, m, thr)
     same(st, one)
-    # and the oracle's execute_stream
-    steps = [[np.asfortranarray(c[..., t]) for c in comps] for t in range(comps[0].shape[-1])]
-    ref = oracle.run_stream(steps, kind, m, thr=thr)
+    # and the oracle's execute_stream (fiber: its one-shot execute, see above)
+    if case in ORACLE_STREAM_ON_GPU_HOST:
+        steps = [[np.asfortranarray(c[..., t]) for c in comps] for t in range(comps[0].shape[-1])]
+        ref = oracle.run_stream(steps, kind, m, thr=thr)
+    else:
+        ref = oracle.run(comps, kind, m, thr=thr)
+    assert len(ref["verts"]) > 0
     assert np.array_equal(st.vertices, ref["verts"])
     assert np.array_equal(st.track_id, ref["track"])
     for d, cells in ref["conn"].items():
@@ -127,3 +135,23 @@ def test_stream_retirement_equals_oneshot(case, retire):
     assert resident[-1] < res_keep[-1]
     if retire == 0:
         assert max(resident[2:]) < res_keep[-1]
+
+
+def test_stream_index_overflow_is_reported(monkeypatch):
+    """Committed node indices are uint32: a stream that would pass the limit fails with
+    FTK2H_E_OVERFLOW instead of wrapping (the limit lowered through
+    FTK2H_STREAM_NODE_LIMIT, read when the stream opens)."""
+    comps = list(synth.cp2d_config(nx=40, ny=36, nt=8, seed=2))
+    mesh = engine.RegularSimplicialMesh(comps[0].shape)
+    n = len(engine.extract(mesh, 0, 2, comps).to_complex().vertices)
+    assert n > 40
+    monkeypatch.setenv("FTK2H_STREAM_NODE_LIMIT", str(n // 2))
+    s = engine.Stream(comps[0].shape[:-1], 0, 2, _lib.F64, 2)
+    with pytest.raises(_lib.Ftk2hError) as e:
+        for t in range(8):
+            s.append([np.asfortranarray(c[..., t]) for c in comps])
+        s.finish()
+    assert e.value.code == 3
+    s.close()
+    monkeypatch.delenv("FTK2H_STREAM_NODE_LIMIT")
+    same(run_stream(comps, 0, 2), engine.extract(mesh, 0, 2, comps).to_complex())

@@ -141,3 +141,33 @@ def test_stream_equals_oneshot():  # test_examples.cpp:259-297 contract
     assert np.array_equal(one["verts"], st["verts"])
     assert np.array_equal(one["track"], st["track"])
     assert np.array_equal(one["conn"][1], st["conn"][1])
+
+
+@pytest.mark.parametrize("case", ["fiber", "contour", "cp3d", "cp3d_float"])
+def test_stream_equals_oneshot_every_predicate(case):
+    """execute_stream (one [t, t+1] slab per step, engine.hpp:214-270) equals execute()
+    on the whole grid for every predicate, lambda bits and all cell dimensions
+    included: the stream restatement the GPU stream tests compare with."""
+    from ftk2_amd import synth
+    if case == "fiber":
+        comps = [synth.scalar_config(n=14, nt=5, ngauss=8, seed=s, dtype=np.float64) for s in (7, 8)]
+        kind, m, thr = oracle.PRED_FIBER, 2, (0.0, 0.0)
+    elif case == "contour":
+        comps = [synth.scalar_config(n=16, nt=6, ngauss=8, seed=1, dtype=np.float64)]
+        kind, m, thr = oracle.PRED_CONTOUR, 1, (0.0, 0.0)
+    elif case == "cp3d":
+        comps, kind, m, thr = synth.cp3d_config(n=14, nt=5, seed=3, noise=0.02), oracle.PRED_CP, 3, (0.0, 0.0)
+    else:
+        comps = synth.gaussians((14, 13, 12), 5, 6, sigma=(8, 24), seed=4, noise=0.02, zero_frac=0.02,
+                                dup_frac=0.02, dtype=np.float32)
+        kind, m, thr = oracle.PRED_CP, 3, (0.0, 0.0)
+    one = oracle.run(comps, kind, m, thr=thr, nthreads=4)
+    steps = [[np.asfortranarray(c[..., t]) for c in comps] for t in range(comps[0].shape[-1])]
+    st = oracle.run_stream(steps, kind, m, thr=thr, nthreads=4)
+    assert len(one["verts"]) > 0
+    for k in ("verts", "track", "bary", "scalar"):
+        assert np.array_equal(one[k], st[k]), k
+    assert np.array_equal(one["lam"].view(np.uint64), st["lam"].view(np.uint64))
+    assert sorted(one["conn"]) == sorted(st["conn"])
+    for d in one["conn"]:
+        assert np.array_equal(one["conn"][d], st["conn"][d])
