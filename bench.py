@@ -137,13 +137,8 @@ def batch_leg(seed, njobs, reps, log_every=256):
     rng = np.random.default_rng(7919 + seed)
     jobs, simp, in_bytes = [], 0, 0
     t0 = time.perf_counter()
-    for i in range(njobs):
-        nx, ny = (int(round(np.exp(rng.uniform(np.log(32), np.log(1024))))) for _ in range(2))
-        nt = int(rng.integers(2, 17))
-        ng = int(rng.integers(4, 65))
-        u, v = synth.gaussian_gradient_device((nx, ny), nt, ng, sigma=(4.0, 10.0), seed=seed * 100003 + i)
-        # index [x, y, t] views of (t, y, x) blocks
-        jobs.append((engine.RegularSimplicialMesh([nx, ny, nt]), [u.permute(2, 1, 0), v.permute(2, 1, 0)]))
+    for (nx, ny, nt), u, v in synth.batch_jobs(njobs, seed):
+        jobs.append((engine.RegularSimplicialMesh([nx, ny, nt]), [u, v]))
         simp += (nx - 1) * (ny - 1) * (nt - 1) * 12
         in_bytes += 2 * nx * ny * nt * 8
     torch.cuda.synchronize()
@@ -201,25 +196,14 @@ def serve_leg(seed, n_req, rates_x, budget_gb):
     multiples of the plain (one request at a time) service rate."""
     import torch
     from ftk2_amd import _lib, engine, serve, synth
-    rng = np.random.default_rng(104729 + seed)
-    sides2, sides3, tmax = [128, 256, 512, 1024, 2048], [64, 128, 256], 32
     t0 = time.perf_counter()
-    pools = {}
-    for n in sides2:
-        pools[(2, n)] = synth.gaussian_gradient_device((n, n), tmax, 16, sigma=(4.0, 10.0), seed=seed * 7 + n)
-    for n in sides3:
-        pools[(3, n)] = synth.gaussian_gradient_device((n, n, n), tmax, 32, sigma=(8.0, 24.0), seed=seed * 11 + n)
+    pools = synth.serving_pools(seed)
     torch.cuda.synchronize()
     log(f"[serve] request pools generated in {time.perf_counter() - t0:.1f} s")
 
     def make(nreq, rs):
         out = []
-        for i in range(nreq):
-            if rs.random() < 0.7:
-                sd, n = 2, int(sides2[rs.integers(len(sides2))])
-            else:
-                sd, n = 3, int(sides3[rs.integers(len(sides3))])
-            T = int(rs.integers(1, tmax + 1))
+        for i, (sd, n, T) in enumerate(synth.serving_mix(nreq, rs)):
             pool = pools[(sd, n)]
             perm = tuple(reversed(range(sd)))
             steps = [[p[t].permute(*perm) for p in pool] for t in range(T)]

@@ -109,3 +109,53 @@ def gaussian_gradient_device(spatial, nt, ngauss, sigma=(4.0, 10.0), amp=(0.5, 1
             comp = comp.reshape(nt, spatial[2], spatial[1], spatial[0])
         out.append(comp.contiguous())
     return out
+
+
+def batch_jobs(njobs=1024, seed=0, device="cuda"):
+    """The batching workload: njobs independent 2-component FP64 gradient fields of
+    moving Gaussians (as cp2d_config), spatial sides log-uniform in [32, 1024] per
+    axis, timesteps uniform in [2, 16], 4-64 Gaussians each, generated on the GPU.
+    Returns [((nx, ny, nt), u, v)] with u, v indexed [x, y, t] (views of (t, y, x)
+    blocks)."""
+    rng = np.random.default_rng(7919 + seed)
+    out = []
+    for i in range(njobs):
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(32), np.log(1024))))) for _ in range(2))
+        nt = int(rng.integers(2, 17))
+        ng = int(rng.integers(4, 65))
+        u, v = gaussian_gradient_device((nx, ny), nt, ng, sigma=(4.0, 10.0), seed=seed * 100003 + i, device=device)
+        out.append(((nx, ny, nt), u.permute(2, 1, 0), v.permute(2, 1, 0)))
+    return out
+
+
+SERVE_SIDES_2D = (128, 256, 512, 1024, 2048)
+SERVE_SIDES_3D = (64, 128, 256)
+SERVE_TMAX = 32
+
+
+def serving_pools(seed=0, device="cuda"):
+    """Field pools of the serving workload, one per request shape: FP64 gradients of
+    16 (2-D, sigma U[4,10]) or 32 (3-D, sigma U[8,24]) moving Gaussians over 32
+    timesteps; a request of T timesteps reads the first T.  Keys (sd, side); values:
+    per-component tensors shaped (T, [nz,] ny, nx)."""
+    pools = {}
+    for n in SERVE_SIDES_2D:
+        pools[(2, n)] = gaussian_gradient_device((n, n), SERVE_TMAX, 16, sigma=(4.0, 10.0), seed=seed * 7 + n,
+                                                 device=device)
+    for n in SERVE_SIDES_3D:
+        pools[(3, n)] = gaussian_gradient_device((n, n, n), SERVE_TMAX, 32, sigma=(8.0, 24.0), seed=seed * 11 + n,
+                                                 device=device)
+    return pools
+
+
+def serving_mix(nreq, rng):
+    """Shapes of the serving workload's requests: 70% 2-D critical points at
+    128^2-2048^2, 30% 3-D at 64^3-256^3, 1-32 timesteps each.  [(sd, side, T)]."""
+    out = []
+    for _ in range(nreq):
+        if rng.random() < 0.7:
+            sd, n = 2, int(SERVE_SIDES_2D[rng.integers(len(SERVE_SIDES_2D))])
+        else:
+            sd, n = 3, int(SERVE_SIDES_3D[rng.integers(len(SERVE_SIDES_3D))])
+        out.append((sd, n, int(rng.integers(1, SERVE_TMAX + 1))))
+    return out

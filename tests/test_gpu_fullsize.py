@@ -1,8 +1,12 @@
 """Full-size workloads: 3D critical points on 256^3 x 4 FP64, contour on 384^3 x 4
 FP32 (isovalue 0), fiber on two 256^3 x 4 FP32 fields (thresholds 0, 0).
 
-The CPU engine needs hours at these sizes, so the results are checked by properties
-that hold at any size:
+The 3D critical-point result is compared whole with the oracle: SHA-256 digests of
+every node's vertices, lambda bits, track ids and every edge, against the digest the
+oracle's full-size mode made offline (tests/golden/make_cp3d_fullsize_digest.py,
+about 12 minutes on 8 cores), which also records the input field's digest.
+
+Every workload is also checked by properties that hold at any size:
   * two runs are bit-identical (keys, lambda, track ids, connectivity);
   * node keys are strictly increasing (canonical order, no duplicates);
   * a seeded sample of reported nodes is re-decided by the oracle's exact SoS test
@@ -20,8 +24,12 @@ import sys
 import numpy as np
 import pytest
 
+import json
+import os
+
 import oracle
 from ftk2_amd import _lib, engine, synth
+from fullsize_digest import complex_digest, field_digest
 
 pytestmark = pytest.mark.gpu
 
@@ -168,6 +176,29 @@ def check_full(case, seed):
     assert sorted(cb) == sorted(conn)
     for dim in conn:
         assert np.array_equal(cb[dim], conn[dim])
+
+
+@pytest.mark.timeout(900)
+def test_cp3d_256x4_fp64_equals_oracle_digest():
+    """The whole FeatureComplex (160,038,948 nodes, 154,512,221 edges) equals the
+    oracle's, bit for bit: node vertices, lambda, track ids and edges."""
+    say("[fullsize] CP3D 256^3 x 4 FP64, seed 0: whole-result digest vs the oracle")
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cp3d_256x4_seed0_digest.json")))
+    comps = synth.cp3d_config(n=256, nt=4, seed=0)
+    assert field_digest(comps) == golden["field"], "the synthetic field differs from the one the digest was made on"
+    r = engine.extract(engine.RegularSimplicialMesh([256, 256, 256, 4]), _lib.PRED_CRITICAL_POINT, 3, comps)
+    del comps
+    n = r.sizes()[0]
+    conn = r.connectivity()[1]
+    say(f"[fullsize] {n} nodes, {len(conn)} edges; hashing")
+
+    def nodes(b, c):
+        verts, lam, _, _, tr = r.node_range(b, c)
+        return verts, lam, tr
+
+    got = complex_digest(n, len(conn), nodes, lambda b, c: conn[b:b + c])
+    for k in ("n_nodes", "n_edges", "vertices", "lambda", "track", "edges"):
+        assert got[k] == golden[k], f"{k}: {got[k]} != oracle {golden[k]}"
 
 
 @pytest.mark.timeout(900)
