@@ -67,6 +67,8 @@ def lib():
         L.oracle_det3_level0.argtypes = [I64P, I64P, I64P, I64P, U64P]
         L.oracle_origin_inside.argtypes = [C.c_int, C.c_int, DP, U64P, C.c_double]
         L.oracle_solve.argtypes = [C.c_int, C.c_int, DP, DP]
+        L.oracle_origin_inside_many.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_double,
+                                                C.c_void_p]
         L.oracle_chains.argtypes = [C.c_int, C.c_int, C.c_void_p]
         L.oracle_encode_simplex_id.restype = C.c_uint64
         L.oracle_encode_simplex_id.argtypes = [C.c_int, U64P, C.c_int, U64P]
@@ -269,6 +271,16 @@ def origin_inside(values, idx, Q=1e6, dtype=1):
     v_, vp = _dp(v.ravel())
     i_, ip = _u64(idx)
     return bool(lib().oracle_origin_inside(K, dtype, vp, ip, Q))
+
+
+def origin_inside_many(values, idx, Q=1e6, dtype=1):
+    """values (n, K+1, K), idx (n, K+1) -> bool (n,)"""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    n, kp1, K = v.shape
+    ix = np.ascontiguousarray(idx, dtype=np.uint64)
+    out = np.zeros(n, np.int32)
+    lib().oracle_origin_inside_many(K, dtype, n, v.ctypes.data, ix.ctypes.data, Q, out.ctypes.data)
+    return out.astype(bool)
 
 
 def solve(values, dtype=1):

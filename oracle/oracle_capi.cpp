@@ -260,6 +260,12 @@ int oracle_origin_inside(int K, int dtype, const double* vals, const uint64_t* i
   return dtype == 0 ? go(float{}) : go(double{});
 }
 
+// origin_inside<K> on n simplices at once: vals[n][K+1][K], idx[n][K+1] -> out[n].
+void oracle_origin_inside_many(int K, int dtype, int64_t n, const double* vals, const uint64_t* idx, double Q,
+                               int* out) {
+  for (int64_t s = 0; s < n; ++s) out[s] = oracle_origin_inside(K, dtype, vals + s * (K + 1) * K, idx + s * (K + 1), Q);
+}
+
 // Returns 1 if the solver succeeds; lambda (K+1 entries) is written either way.
 int oracle_solve(int K, int dtype, const double* vals, double* lambda) {
   auto go = [&](auto tag) -> int {
