@@ -15,6 +15,10 @@ FTK2H_OK = 0
 STATUS = {0: "OK", 1: "E_INVALID", 2: "E_UNSUPPORTED", 3: "E_OVERFLOW", 4: "E_OOM", 5: "E_HIP", 6: "E_AGAIN"}
 E_AGAIN = 6
 PRED_CRITICAL_POINT, PRED_CONTOUR, PRED_FIBER = 0, 1, 2
+CLASSIFY_GRADIENT, CLASSIFY_FLOW = 1, 2
+CP_TYPES = {0: "unclassified", 1: "minimum", 2: "maximum", 3: "saddle", 4: "saddle-1", 5: "saddle-2",
+            16: "source", 17: "sink", 18: "saddle", 19: "spiral source", 20: "spiral sink", 21: "center",
+            255: "degenerate"}
 F32, F64 = 0, 1
 
 
@@ -105,6 +109,8 @@ SIGNATURES = {
     "ftk2h_result_add_attributes": (C.c_int, [_P, _P, C.POINTER(MeshDesc), C.POINTER(AttributeDesc), C.c_int,
                                               _P]),
     "ftk2h_result_copy_attributes": (C.c_int, [_P, _P]),
+    "ftk2h_result_classify": (C.c_int, [_P, _P, C.POINTER(MeshDesc), C.POINTER(FieldView), C.c_int, C.c_int, _P]),
+    "ftk2h_result_copy_types": (C.c_int, [_P, _P]),
     "ftk2h_server_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
     "ftk2h_server_open": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint64), C.POINTER(PredicateDesc), C.c_int,
                                     C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -64,7 +64,7 @@ struct DevBuf {
   }
 };
 
-constexpr int NPHASE = 7;
+constexpr int NPHASE = 8;
 
 // Cache of device blocks for result arrays: a freed result returns its blocks
 // here and the next extraction of a similar size takes them back instead of
@@ -161,6 +161,7 @@ struct ftk2h_result {
   float* scalar = nullptr;
   float scalar_value = 0.f;  // FeatureElement::scalar of every node when `scalar` is null
   float* attrs = nullptr;  // n_nodes * 16 (FeatureElement::attributes), when requested
+  uint32_t* types = nullptr;  // n_nodes (FeatureElement::type), when classified
   int n_conn = 0;
   int conn_dim[3] = {0, 0, 0};
   uint64_t conn_cells[3] = {0, 0, 0};
@@ -187,6 +188,7 @@ struct ftk2h_result {
   }
   ~ftk2h_result() {
     release(attrs);  // never part of a batch's shared memory
+    release(types);
     if (arena) return;
     release(keys);
     release(lambda);
@@ -432,25 +434,26 @@ int run_pipeline(RunArgs& A) {
   const int64_t nb = A.nb;
   auto rec = [&](int i) { return hipEventRecord(ctx->ev[i], st); };
   uint32_t* scan_sums = A.fixed ? A.fixed->scan_sums : static_cast<uint32_t*>(ctx->scan_sums.p);
+  // events around each kernel (phases: classify, stage 2, node scan, offsets, cell scan
+  // + sync, emit, link, compress)
   HIP_TRY(rec(0));
-  if (nb > 0) {
-    k_classify<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
-    k_stage2<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
-  }
+  if (nb > 0) k_classify<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(1));
-  HIP_TRY(scan_u32(P.bsum, P.bsum, (uint64_t)nb, scan_sums, st));
+  if (nb > 0) k_stage2<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(2));
-  if (nb > 0) k_offsets<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
+  HIP_TRY(scan_u32(P.bsum, P.bsum, (uint64_t)nb, scan_sums, st));
   HIP_TRY(hipGetLastError());
   HIP_TRY(rec(3));
+  if (nb > 0) k_offsets<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(rec(4));
   HIP_TRY(scan_u32(P.ebsum, P.ebsum, (uint64_t)nb, scan_sums, st));
   HIP_TRY(hipGetLastError());
   uint32_t totals[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(&totals[0], P.bsum + nb, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(&totals[1], P.ebsum + nb, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(rec(4));
   HIP_TRY(hipStreamSynchronize(st));
   const uint64_t nn = totals[0], nc = totals[1];
   if (A.node_cap && nn > A.node_cap)
@@ -517,9 +520,10 @@ int run_pipeline(RunArgs& A) {
   P.track = R->track;
   P.n_nodes = nn;
   P.n_cells = nc;
+  HIP_TRY(rec(5));
   if (nb > 0 && nn) k_emit<KIND, D, M, T><<<dim3((unsigned)nb), dim3(BLOCK), 0, st>>>(P);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(rec(5));
+  HIP_TRY(rec(6));
   if constexpr (NV > 0) {
     if (nc) {
       // at most nc * (NV - 1) pairs; threads beyond the device-side count exit
@@ -528,12 +532,12 @@ int run_pipeline(RunArgs& A) {
       HIP_TRY(hipGetLastError());
     }
   }
-  HIP_TRY(rec(6));
+  HIP_TRY(rec(7));
   if (nn) {
     launch_compress(parent, R->track, nn, st);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(rec(7));
+  HIP_TRY(rec(8));
   if constexpr (CellKind<KIND, D, M>::value == 2) {
     // star-fan triangles: global sort + unique (FeatureComplex::sort)
     if (nc) {

@@ -191,8 +191,10 @@ class Result:
         return n.value, m.value, nc.value
 
     def phase_ms(self):
-        buf = (C.c_float * 7)()
-        _lib.check(_lib.lib().ftk2h_result_phase_ms(self.h, buf, 7))
+        """Device ms per phase: k_classify, k_stage2, node scan, k_offsets, cell scan +
+        sync, k_emit, k_link_pairs, k_compress (HIP events on the extraction's stream)."""
+        buf = (C.c_float * 8)()
+        _lib.check(_lib.lib().ftk2h_result_phase_ms(self.h, buf, 8))
         return list(buf)
 
     def device_view(self):
@@ -206,6 +208,24 @@ class Result:
         if n:
             _lib.check(_lib.lib().ftk2h_result_copy_keys(self.h, k.ctypes.data))
         return k
+
+    def classify(self, mesh, comps, mode: int, device: int = 0):
+        """FeatureElement::type of every critical point (ftk2h_result_classify): mode
+        _lib.CLASSIFY_GRADIENT (Hessian eigenvalue signs) or _lib.CLASSIFY_FLOW
+        (Jacobian eigenvalues); comps: the extraction's field components."""
+        comps = list(comps)
+        views = (_lib.FieldView * len(comps))(*[field_view(c) for c in comps])
+        md = mesh.desc()
+        _lib.check(_lib.lib().ftk2h_result_classify(context(device), self.h, C.byref(md), views, len(comps),
+                                                    int(mode), None))
+        return self.types()
+
+    def types(self):
+        n, m, _ = self.sizes()
+        out = np.zeros(n, np.uint32)
+        if n:
+            _lib.check(_lib.lib().ftk2h_result_copy_types(self.h, out.ctypes.data))
+        return out
 
     def add_attributes(self, mesh, specs, device: int = 0):
         """specs: [(slot, [arrays indexed by local coordinates])] (several arrays: norm)."""

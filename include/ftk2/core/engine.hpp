@@ -149,6 +149,16 @@ class SimplicialEngine {
       detail::check(ftk2h_extract_regular(detail::context(device_), &md, &pd, views.data(), (int)views.size(),
                                           has_scalar ? &scalar : nullptr, nullptr, &result_, nullptr));
       add_attributes(data, reg.get(), md);
+      if constexpr (detail::pred_traits<PredicateType>::kind == FTK2H_PRED_CRITICAL_POINT) {
+        if (!predicate_.field_type.empty()) {
+          const int mode = predicate_.field_type == "gradient" ? FTK2H_CLASSIFY_GRADIENT
+                           : predicate_.field_type == "flow"   ? FTK2H_CLASSIFY_FLOW
+                                                               : 0;
+          if (!mode) throw std::invalid_argument("field_type must be 'gradient' or 'flow'");
+          detail::check(ftk2h_result_classify(detail::context(device_), result_, &md, views.data(), (int)views.size(),
+                                              mode, nullptr));
+        }
+      }
     }
   }
 
@@ -308,6 +318,8 @@ class SimplicialEngine {
     if (n) detail::check(ftk2h_result_copy_nodes(result_, verts.data(), nullptr, bary.data(), scal.data(), track.data()));
     std::vector<float> attrs(n * 16);
     if (n) detail::check(ftk2h_result_copy_attributes(result_, attrs.data()));
+    std::vector<uint32_t> types(n);
+    if (n) detail::check(ftk2h_result_copy_types(result_, types.data()));
     cx.vertices.resize(n);
     for (uint64_t i = 0; i < n; ++i) {
       FeatureElement& el = cx.vertices[i];
@@ -319,7 +331,7 @@ class SimplicialEngine {
       }
       el.geometry_type = FeatureGeometryType::Point;
       el.track_id = track[i];
-      el.type = 0;
+      el.type = types[i];  // 0 unless the predicate asked for a classification
       el.scalar = scal[i];
     }
     for (int c = 0; c < nconn; ++c) {

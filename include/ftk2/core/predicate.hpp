@@ -29,6 +29,12 @@ struct CriticalPointPredicate : public Predicate<M, T> {
   std::string scalar_var_name;
   bool use_multicomponent = true;
   std::vector<AttributeSpec> attributes;
+  // Not a field of the reference's predicate: "gradient" (Hessian classification:
+  // min / max / saddles) or "flow" (Jacobian classification: source / sink / saddle /
+  // spiral), the values of the reference's tracking configuration
+  // (high_level/tracking_config.hpp:84-88), sets FeatureElement::type
+  // (ftk2h_cp_type in ftk2h.h); empty keeps type = 0 as the reference's engine does.
+  std::string field_type;
 };
 
 template <typename T = double>

@@ -147,8 +147,10 @@ int ftk2h_result_copy_node_range(const ftk2h_result* r, uint64_t begin, uint64_t
 int ftk2h_result_copy_keys(const ftk2h_result* r, uint64_t* keys);
 int ftk2h_result_copy_conn(const ftk2h_result* r, int i, uint32_t* indices);
 int ftk2h_result_device_view(const ftk2h_result* r, ftk2h_device_view* out);
-/* Per-phase device time in ms of the extraction that produced r (n entries,
- * phases: classify, scan nodes, offsets+cell count, scan cells, emit, link, compress). */
+/* Per-phase device time in ms of the extraction that produced r (n entries, HIP events
+ * around each kernel on the extraction's stream; phases: classification stage 1,
+ * stage 2, node scan, offsets + cell count, cell scan + host sync, emit, link,
+ * compress). */
 int ftk2h_result_phase_ms(const ftk2h_result* r, float* ms, int n);
 void ftk2h_result_free(ftk2h_result* r);
 
@@ -178,6 +180,40 @@ int ftk2h_result_copy_attributes(const ftk2h_result* r, float* out);
  * extruded (time_id_stride != 0), else 0.  Host buffers of 3 * n_nodes and
  * n_nodes floats. */
 int ftk2h_result_copy_positions(const ftk2h_result* r, float* xyz, float* time);
+
+/* ---- Critical-point classification (opt-in) ---------------------------------------
+ * No counterpart in the reference's engine, whose CriticalPointPredicate always sets
+ * FeatureElement::type = 0 (predicate.hpp:80); its tracking configuration describes
+ * Hessian classification of gradient fields (min / max / saddles) and Jacobian
+ * classification of flow fields (source / sink / spiral), high_level/
+ * tracking_config.hpp:84-88.  For critical points of an M-component field on an
+ * M-dimensional space + time (M = 1..3): the spatial Jacobian J of the field at each
+ * vertex of the node's simplex by central differences over the extraction's local
+ * grid (one-sided at its faces, per timestep), combined as sum_k lambda_k J_k with the
+ * node's barycentric coordinates, in double.  GRADIENT classifies the symmetric part
+ * (J + J^T)/2 by the signs of its eigenvalues; FLOW classifies J's eigenvalues
+ * (real parts, and a complex pair as a spiral).  comps: the views of the extraction.
+ * Types stay 0 unless this is called; ftk2h_result_copy_types reads them. */
+typedef enum { FTK2H_CLASSIFY_GRADIENT = 1, FTK2H_CLASSIFY_FLOW = 2 } ftk2h_classify_mode;
+typedef enum {
+  FTK2H_CP_UNCLASSIFIED = 0,
+  FTK2H_CP_MINIMUM = 1,       /* gradient: every eigenvalue positive */
+  FTK2H_CP_MAXIMUM = 2,       /* gradient: every eigenvalue negative */
+  FTK2H_CP_SADDLE = 3,        /* gradient, M = 2: one of each sign */
+  FTK2H_CP_SADDLE_1 = 4,      /* gradient, M = 3: one negative eigenvalue */
+  FTK2H_CP_SADDLE_2 = 5,      /* gradient, M = 3: two negative eigenvalues */
+  FTK2H_CP_SOURCE = 16,       /* flow: every real part positive, real eigenvalues */
+  FTK2H_CP_SINK = 17,         /* flow: every real part negative, real eigenvalues */
+  FTK2H_CP_FLOW_SADDLE = 18,  /* flow: real parts of both signs */
+  FTK2H_CP_SPIRAL_SOURCE = 19,/* flow: complex pair, every real part positive */
+  FTK2H_CP_SPIRAL_SINK = 20,  /* flow: complex pair, every real part negative */
+  FTK2H_CP_CENTER = 21,       /* flow: a purely imaginary pair */
+  FTK2H_CP_DEGENERATE = 255   /* a zero eigenvalue (det J = 0) */
+} ftk2h_cp_type;
+int ftk2h_result_classify(ftk2h_ctx* ctx, ftk2h_result* r, const ftk2h_mesh_desc* mesh,
+                          const ftk2h_field_view* comps, int ncomp, int mode, void* hip_stream);
+/* n_nodes types (FeatureElement::type); zeros when not classified. */
+int ftk2h_result_copy_types(const ftk2h_result* r, uint32_t* types);
 
 /* ---- Streaming (SimplicialEngine::execute_stream, engine.hpp:214-270) ----------
  * Timesteps of a spatial grid arrive one at a time (ftk2h_stream_append, views

@@ -366,6 +366,72 @@ static void extruded_2d() {
   std::printf("extruded_2d: %zu vertices\n", cx.vertices.size());
 }
 
+// The reference's default critical-point input: one multi-component array [M, x, y, t]
+// named by vector_var_name (use_multicomponent = true, engine.hpp:1096-1124), equal to
+// the legacy separate arrays on the same values; and the opt-in classification
+// (field_type, not a reference predicate field) of a moving quadratic potential's
+// gradient: every node a minimum (type 1), types 0 without it.
+static void multicomponent_and_classification() {
+  const int W = 26, H = 22, Tn = 5;
+  ftk::ndarray<double> u({(size_t)W, (size_t)H, (size_t)Tn}), v({(size_t)W, (size_t)H, (size_t)Tn});
+  ftk::ndarray<double> uv({(size_t)2, (size_t)W, (size_t)H, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        const double a = std::sin(0.37 * x + 0.11 * t) * std::cos(0.29 * y), b = std::cos(0.41 * x) * std::sin(0.33 * y - 0.07 * t);
+        u.f(x, y, t) = a;
+        v.f(x, y, t) = b;
+        uv.f(0, x, y, t) = a;
+        uv.f(1, x, y, t) = b;
+      }
+  std::map<std::string, ftk::ndarray<double>> data = {{"U", u}, {"V", v}, {"UV", uv}};
+  auto mesh = std::make_shared<RegularSimplicialMesh>(std::vector<uint64_t>{(uint64_t)W, (uint64_t)H, (uint64_t)Tn});
+  CriticalPointPredicate<2, double> legacy;
+  legacy.var_names[0] = "U";
+  legacy.var_names[1] = "V";
+  CriticalPointPredicate<2, double> multi;
+  multi.vector_var_name = "UV";
+  SimplicialEngine<double, CriticalPointPredicate<2, double>> e1(mesh, legacy), e2(mesh, multi);
+  e1.execute(data);
+  e2.execute(data);
+  auto c1 = e1.get_complex(), c2 = e2.get_complex();
+  CHECK_EQ(c1.vertices.size() > 10, true);
+  CHECK_EQ(c1.vertices.size(), c2.vertices.size());
+  for (size_t i = 0; i < c1.vertices.size() && i < c2.vertices.size(); ++i) {
+    CHECK_EQ(c1.vertices[i].simplex == c2.vertices[i].simplex, true);
+    CHECK_EQ(c1.vertices[i].track_id, c2.vertices[i].track_id);
+    CHECK_EQ(c1.vertices[i].barycentric_coords[0][0] == c2.vertices[i].barycentric_coords[0][0], true);
+    CHECK_EQ(c1.vertices[i].type, 0u);
+  }
+  CHECK_EQ(c1.connectivity.size(), c2.connectivity.size());
+  if (!c1.connectivity.empty() && !c2.connectivity.empty())
+    CHECK_EQ(c1.connectivity[0].indices == c2.connectivity[0].indices, true);
+  check_complex(c2);
+  // f = (x - cx)^2 + 1.5 (y - cy)^2 with a moving centre: its gradient's critical
+  // points are minima
+  ftk::ndarray<double> g({(size_t)2, (size_t)W, (size_t)H, (size_t)Tn});
+  for (int t = 0; t < Tn; ++t)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        g.f(0, x, y, t) = 2.0 * (x - (11.37 + 0.41 * t));
+        g.f(1, x, y, t) = 3.0 * (y - (9.23 - 0.27 * t));
+      }
+  std::map<std::string, ftk::ndarray<double>> gdata = {{"G", g}};
+  CriticalPointPredicate<2, double> gp;
+  gp.vector_var_name = "G";
+  gp.field_type = "gradient";
+  SimplicialEngine<double, CriticalPointPredicate<2, double>> e3(mesh, gp);
+  e3.execute(gdata);
+  auto c3 = e3.get_complex();
+  CHECK_EQ(c3.vertices.size() > 0, true);
+  for (const auto& el : c3.vertices) CHECK_EQ(el.type, (uint32_t)FTK2H_CP_MINIMUM);
+  gp.field_type = "flow";  // the same field read as a flow: a source
+  SimplicialEngine<double, CriticalPointPredicate<2, double>> e4(mesh, gp);
+  e4.execute(gdata);
+  for (const auto& el : e4.get_complex().vertices) CHECK_EQ(el.type, (uint32_t)FTK2H_CP_SOURCE);
+  std::printf("multicomponent: %zu vertices, classified: %zu\n", c2.vertices.size(), c3.vertices.size());
+}
+
 int main() {
   extruded_2d();
   attributes_3d();
@@ -374,6 +440,7 @@ int main() {
   critical_point_3d();
   critical_point_2d();
   streaming();
+  multicomponent_and_classification();
   std::printf("%d failures\n", failures);
   return failures;
 }

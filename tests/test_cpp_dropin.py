@@ -42,3 +42,4 @@ def test_cpp_examples_on_gpu():
     assert "streaming_retired:" in p.stdout  # the same with retirement after every slab
     assert "attributes_3d:" in p.stdout
     assert "extruded_2d:" in p.stdout  # ExtrudedSimplicialMesh over a regular base
+    assert "multicomponent:" in p.stdout  # vector_var_name (the default input mode) and classification
