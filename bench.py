@@ -25,15 +25,32 @@ N_TYPES = 60  # tetrahedra per 4-D hypercube
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "cp3d_256x4_fp64_hbm_traffic.md")
 
 
-def classify_traffic(n, nt):
-    """HBM bytes of one classification (k_classify + k_stage2) measured by the rocprofv3
-    FETCH_SIZE/WRITE_SIZE passes recorded in the committed summary (256^3 x 4 only;
-    None otherwise)."""
+def kernel_source_hash():
+    """SHA-256 over the library's kernel and host sources (ftk2_amd/csrc), recorded in
+    the PMC traffic summary so that a summary of other kernels is never used."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "ftk2_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".hpp", ".cpp", ".h")):
+            h.update(name.encode())
+            h.update(open(os.path.join(d, name), "rb").read())
+    return h.hexdigest()
+
+
+def measured_traffic(kernel, n, nt):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated) from the
+    committed rocprofv3 PMC summary of the 256^3 x 4 workload, or None when there is no
+    summary for these sources (its recorded source hash differs)."""
     if (n, nt) != (256, 4) or not os.path.exists(TRAFFIC_SUMMARY):
-        return None
+        return None, "no PMC summary for this workload"
     import re
-    m = re.search(r"^classify_traffic_bytes:\s*(\d+)", open(TRAFFIC_SUMMARY).read(), re.M)
-    return int(m.group(1)) if m else None
+    txt = open(TRAFFIC_SUMMARY).read()
+    m = re.search(r"^source_sha256:\s*([0-9a-f]{64})", txt, re.M)
+    if not m or m.group(1) != kernel_source_hash():
+        return None, "PMC summary made on other kernel sources"
+    m = re.search(r"^" + re.escape(kernel) + r"_traffic_bytes:\s*(\d+)", txt, re.M)
+    return (int(m.group(1)), "rocprofv3 PMC summary") if m else (None, "kernel not in the PMC summary")
 
 
 def log(*a):
@@ -100,7 +117,9 @@ def scalar_workload(args, name):
         f"cells={n_cells}")
     n_simp = (n - 1) ** 3 * (nt - 1) * ntypes
     in_bytes = len(comps) * n ** 3 * nt * 4
-    achieved = in_bytes / (float(ph[0]) * 1e-3) / 1e9
+    # k_emit (phase 5) writes node key + lambda and the cells
+    node_bytes = int(n_nodes) * (8 + (m + 1) * 4) + int(n_cells) * 4 * (3 if m == 2 else 4)
+    achieved = node_bytes / (float(ph[5]) * 1e-3) / 1e9
     out = {
         "metric": f"simplices/s, {name} {n}^3x{nt} FP32",
         "value": n_simp / dt, "unit": "simplices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -108,9 +127,9 @@ def scalar_workload(args, name):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{name}_{n}^3x{nt}_fp32", "fields": len(comps), "gaussians": 64,
                    "thresholds": list(thr[:len(comps)]), "nodes": int(n_nodes), "cells": int(n_cells)},
-        "roofline": {"bound": "hbm", "kernel": "k_classify+k_stage2", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_emit", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": int(in_bytes)},
+                     "algorithmic_bytes": int(node_bytes), "kernel_ms": float(ph[5])},
         "pipeline": {"phase_ms": [float(x) for x in ph]},
     }
     if not args.no_cpu:
@@ -354,15 +373,20 @@ def main():
     # (two u32) + track id (u64), as the reference's record layout would count it
     total_bytes = in_bytes + n_nodes * (8 + 4 * 8) + n_edges * 8 + n_nodes * 8
     kernel_ms = float(ph.sum())
-    classify_ms = float(ph[0])
-    emit_ms = float(ph[4])
-    # classify = stage 1 (k_classify) + stage 2 (k_stage2), the predicate evaluation
-    dominant = "k_classify+k_stage2" if classify_ms >= emit_ms else "k_emit"
-    if dominant == "k_classify+k_stage2":
-        dom_bytes, dom_ms = in_bytes, classify_ms
-    else:
-        dom_bytes, dom_ms = n_nodes * (8 + 4 * 8) + n_edges * 8, emit_ms
+    # the largest single kernel (HIP events around each kernel on its stream) and its
+    # algorithmic bytes: k_classify reads the inputs once; k_emit writes node key +
+    # lambda and the edges; k_compress writes the track ids; stage 2 and the union-find
+    # hooking have no algorithmic bytes of their own (they re-read what the first stage
+    # decided) and are never named
+    kernels = {"k_classify": (float(ph[0]), in_bytes),
+               "k_stage2": (float(ph[1]), None),
+               "k_emit": (float(ph[5]), n_nodes * (8 + 4 * 8) + n_edges * 8),
+               "k_link_pairs": (float(ph[6]), None),
+               "k_compress": (float(ph[7]), n_nodes * 8)}
+    dominant = max((k for k in kernels if kernels[k][1] is not None), key=lambda k: kernels[k][0])
+    dom_ms, dom_bytes = kernels[dominant]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(dominant, n, nt)
     out = {
         "metric": "simplices/s + %HBM roofline, 3D CP 256^3x4 FP64; p50/p99 latency in serve loop",
         "value": n_simp / dt,
@@ -380,12 +404,12 @@ def main():
                    "mesh": [n, n, n, nt], "gaussians": 32, "noise": 1e-3, "zero_frac": 0.01,
                    "dup_frac": 0.01, "seed": args.seed, "nodes": int(n_nodes), "edges": int(n_edges)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": classify_traffic(n, nt) if dominant == "k_classify+k_stage2" else None,
-                     "algorithmic_bytes": int(dom_bytes)},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_bytes": int(dom_bytes),
+                     "kernel_ms": dom_ms},
         "pipeline": {"kernel_ms": kernel_ms, "phase_ms": [float(x) for x in ph],
-                     "phases": ["classify", "scan_nodes", "offsets+cell_count", "scan_cells+sync",
-                                "emit", "link", "compress"],
+                     "phases": ["k_classify", "k_stage2", "scan_nodes", "k_offsets", "scan_cells+sync",
+                                "k_emit", "k_link_pairs", "k_compress"],
                      "algorithmic_bytes": int(total_bytes), "input_bytes": int(in_bytes),
                      "hbm_frac_total": total_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "hbm_frac_input_only": in_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
