@@ -305,6 +305,40 @@ def encode_simplex_id(gdims, verts):
     return int(lib().oracle_encode_simplex_id(d, g_p, len(verts) - 1, v_p))
 
 
+def positions(verts, bary, gdims, time_stride=0):
+    """Point positions and "Time" as the reference's VTK writer computes them
+    (utils/vtk.hpp:26-59, complex_to_vtk_points): per node, phys_pos[j] +=
+    (double)barycentric_coords[0][i] * coordinate j of vertex i, for the node's
+    vertices in order, over every mesh dimension; the point is phys_pos[0..2] stored as
+    float (vtkPoints), Time = (float) phys_pos[last] when the mesh has a time axis beyond
+    its spatial dimension (an ExtrudedSimplicialMesh), else 0.  Vertex coordinates are
+    the global grid indices (mesh.hpp:420-451 id_to_grid_index; extruded: layer =
+    id / n_spatial_verts_, mesh.hpp:711-717).  Returns ((n, 3) float32, (n,) float32)."""
+    verts = np.asarray(verts, dtype=np.uint64)
+    n, nv = verts.shape
+    d = len(gdims)
+    coords = np.zeros((n, nv, d), np.float64)
+    rest = verts.copy()
+    sd = d
+    if time_stride:
+        coords[:, :, d - 1] = (rest // np.uint64(time_stride)).astype(np.float64)
+        rest = rest % np.uint64(time_stride)
+        sd = d - 1
+    for j in range(sd):
+        coords[:, :, j] = (rest % np.uint64(gdims[j])).astype(np.float64)
+        rest = rest // np.uint64(gdims[j])
+    phys = np.zeros((n, d), np.float64)
+    for i in range(nv):
+        b = np.asarray(bary[:, i], dtype=np.float32).astype(np.float64)
+        for j in range(d):
+            phys[:, j] = phys[:, j] + b * coords[:, i, j]
+    pts = np.zeros((n, 3), np.float32)
+    for j in range(min(3, d)):
+        pts[:, j] = phys[:, j].astype(np.float32)
+    time = phys[:, d - 1].astype(np.float32) if time_stride else np.zeros(n, np.float32)
+    return pts, time
+
+
 def attributes(verts, bary, specs, gdims, offset, dtype):
     """Attribute values of FeatureElement::attributes (engine.hpp:1144-1201).
 

@@ -41,6 +41,8 @@ def compare(name, comps, kind, m, thr=(0.0, 0.0)):
     p, t = res.positions(with_time=True)
     ep, et = vtk.positions(cx.vertices, cx.bary, mesh.global_dims, mesh.time_stride, with_time=True)
     assert np.array_equal(p, ep) and np.array_equal(t, et)
+    op, ot = oracle.positions(cx.vertices, cx.bary, mesh.global_dims, mesh.time_stride)  # utils/vtk.hpp:26-59
+    assert np.array_equal(p, op) and np.array_equal(t, ot)
     return cx
 
 

@@ -1,11 +1,13 @@
 """Node positions computed on the device (ftk2h_result_copy_positions) equal the
-reference VTK writer's formula (utils/vtk.hpp:26-49) as restated in ftk2_amd.vtk,
-bit for bit, and a file written from them reads back with those points."""
+reference VTK writer's formula (utils/vtk.hpp:26-59) as the oracle restates it
+(oracle.positions), bit for bit, as does the host writer's ftk2_amd.vtk.positions, and
+a file written from them reads back with those points."""
 import sys
 
 import numpy as np
 import pytest
 
+import oracle
 from ftk2_amd import engine, synth, vtk
 
 pytestmark = pytest.mark.gpu
@@ -34,6 +36,8 @@ def test_device_positions(case, tmp_path):
     cx = res.to_complex()
     assert len(cx.vertices) > 0
     p = res.positions()
+    want, _ = oracle.positions(cx.vertices, cx.bary, mesh.global_dims)
+    assert np.array_equal(p, want)
     assert np.array_equal(p, vtk.positions(cx.vertices, cx.bary, mesh.global_dims))
     vtk.write_vtu(cx, mesh.global_dims, tmp_path / "r.vtu", points=p)
     r = vtk.read(tmp_path / "r.vtu")

@@ -47,3 +47,28 @@ def test_norm_of_one_component_is_absolute_value_and_slot_overwrite():
                           (0, 0, 0), np.float64)
     b = oracle.attributes(r["verts"], r["bary"], [(2, [np.abs(u)]), (5, [v])], u.shape, (0, 0, 0), np.float64)
     assert np.array_equal(a, b)
+
+
+def test_positions_restatement_pins():
+    """oracle.positions (utils/vtk.hpp:26-59): a node whose barycentric weight is all
+    on one vertex sits on that vertex; the centroid of a unit-offset triangle is exact;
+    points lie in the simplex's bounding box; Time is the layer only on extruded
+    meshes; a coordinate beyond the third is not a point coordinate."""
+    g = (7, 5, 4)
+    v = np.array([[0 + 7 * 1 + 35 * 2, 1 + 7 * 1 + 35 * 2, 1 + 7 * 2 + 35 * 2]], np.uint64)  # (0,1,2),(1,1,2),(1,2,2)
+    p, t = oracle.positions(v, np.array([[1.0, 0.0, 0.0]], np.float32), g)
+    assert p.tolist() == [[0.0, 1.0, 2.0]] and t.tolist() == [0.0]
+    p, _ = oracle.positions(v, np.array([[0.25, 0.25, 0.5]], np.float32), g)
+    assert p.tolist() == [[0.75, 1.5, 2.0]]
+    rng = np.random.default_rng(0)
+    b = rng.dirichlet(np.ones(3), size=50).astype(np.float32)
+    p, _ = oracle.positions(np.repeat(v, 50, axis=0), b, g)
+    assert np.all(p >= np.array([0, 1, 2]) - 1e-6) and np.all(p <= np.array([1, 2, 2]) + 1e-6)
+    # extruded: id = spatial + layer * stride (stride 40 > 7 * 5); Time = layer coordinate
+    v4 = np.array([[3 + 7 * 2 + 40 * 1, 4 + 7 * 2 + 40 * 2]], np.uint64)
+    p, t = oracle.positions(v4, np.array([[0.5, 0.5]], np.float32), (7, 5, 3), time_stride=40)
+    assert p.tolist() == [[3.5, 2.0, 1.5]] and t.tolist() == [1.5]
+    # a 4-D regular mesh: the point is (x, y, z), Time 0
+    v5 = np.array([[1 + 3 * (1 + 3 * (1 + 3 * 2))]], np.uint64)
+    p, t = oracle.positions(v5, np.array([[1.0]], np.float32), (3, 3, 3, 3))
+    assert p.tolist() == [[1.0, 1.0, 1.0]] and t.tolist() == [0.0]
