@@ -57,24 +57,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(comps, n_sample):
-    """The oracle (CPU restatement of the reference engine) on a bounded sample: the
-    first n_sample^3 x 4 block of the same field, all host threads."""
-    import oracle
-    sub = [np.asfortranarray(c[:n_sample, :n_sample, :n_sample, :]) for c in comps]
-    threads = os.cpu_count() or 1
+def host_info():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
     try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
         pass
-    threads = min(threads, 16)
-    t0 = time.perf_counter()
-    r = oracle.run(sub, oracle.PRED_CP, 3, nthreads=threads)
-    dt = time.perf_counter() - t0
-    n_simp = (n_sample - 1) ** 3 * (comps[0].shape[3] - 1) * N_TYPES
-    return {"value": n_simp / dt, "unit": "simplices/s", "cores": threads, "kind": "port",
-            "sample": f"{n_sample}^3x{comps[0].shape[3]} corner block of the same field "
-                      f"({n_simp} tets, {len(r['verts'])} nodes, {dt:.2f} s)"}
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": affinity}
+
+
+def cpu_baseline(comps, n_sample, n_one):
+    """The oracle (CPU restatement of the reference engine) on bounded samples of the
+    same field: the first n_sample^3 x T block with the host's CPU share (all CPUs this
+    process may use, at most 16: the GPU host's share per GPU), and the first
+    n_one^3 x T block on one core."""
+    import oracle
+    info = host_info()
+    nt = comps[0].shape[3]
+
+    def timed(n, threads):
+        sub = [np.asfortranarray(c[:n, :n, :n, :]) for c in comps]
+        t0 = time.perf_counter()
+        r = oracle.run(sub, oracle.PRED_CP, 3, nthreads=threads)
+        dt = time.perf_counter() - t0
+        n_simp = (n - 1) ** 3 * (nt - 1) * N_TYPES
+        return n_simp / dt, f"{n}^3x{nt} corner block of the same field ({n_simp} tets, {len(r['verts'])} nodes, {dt:.2f} s)"
+
+    threads = min(info["affinity"], 16)
+    v_all, s_all = timed(n_sample, threads)
+    v_one, s_one = timed(n_one, 1)
+    return {"value": v_all, "unit": "simplices/s", "cores": threads, "kind": "port", "sample": s_all,
+            "one_core": {"value": v_one, "cores": 1, "sample": s_one}, "host": info}
 
 
 SCALAR_WORKLOADS = {
@@ -143,7 +164,7 @@ def scalar_workload(args, name):
         ns = (s - 1) ** 3 * (nt - 1) * ntypes
         out["cpu_baseline"] = {"value": ns / cdt, "unit": "simplices/s", "cores": threads, "kind": "port",
                                "sample": f"{s}^3x{nt} corner block of the same field(s) ({ns} simplices, "
-                                         f"{len(ref['verts'])} nodes, {cdt:.2f} s)"}
+                                         f"{len(ref['verts'])} nodes, {cdt:.2f} s)", "host": host_info()}
     return out
 
 
@@ -302,6 +323,7 @@ def main():
     ap.add_argument("--nt", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample-one", type=int, default=32, help="side of the one-core CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 1024-job batching leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
@@ -425,8 +447,9 @@ def main():
         out["p99_ms"] = out["serving"]["p99_ms"]
         torch.cuda.empty_cache()
     if not args.no_cpu:
-        log(f"[bench] cpu baseline on a {args.cpu_sample}^3 x {nt} sample")
-        out["cpu_baseline"] = cpu_baseline(comps, args.cpu_sample)
+        log(f"[bench] cpu baseline on a {args.cpu_sample}^3 x {nt} sample and a {args.cpu_sample_one}^3 x {nt} "
+            f"one-core sample")
+        out["cpu_baseline"] = cpu_baseline(comps, args.cpu_sample, args.cpu_sample_one)
     print(json.dumps(out), flush=True)
 
 

@@ -137,6 +137,17 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ftk2_amd.build` "
                                "(there is no CPU fallback)")
+        # One HIP runtime per process.  PyTorch-ROCm carries its own libamdhip64 and
+        # libhsa-runtime64 (SONAMEs libamdhip64.so.7 / libhsa-runtime64.so.1, but its
+        # libraries ask for them as libamdhip64.so / libhsa-runtime64.so).  Loaded first,
+        # this library would bind the system copies, a later `import torch` would map a
+        # second HIP and HSA runtime beside them, and torch's device initialisation would
+        # then find no GPU.  With torch imported first, libamdhip64.so.7 resolves by its
+        # SONAME to the runtime torch already mapped.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

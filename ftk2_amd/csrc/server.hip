@@ -13,6 +13,7 @@
 
 #include <deque>
 #include <map>
+#include <thread>
 #include <tuple>
 
 using namespace ftk2h;
@@ -60,7 +61,11 @@ struct ftk2h_server {
   std::map<BatchKey, ftk2h_batch*> batches;
   DevBuf counts;                    // per-slab lower-plane node counts of a step
   std::vector<uint32_t> host_counts;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;         // single-mesh slabs
+  hipStream_t st_batch = nullptr;   // batched passes, concurrent with them
+  // nodes per slab vertex, the largest seen in a served slab so far (sizes the
+  // reservations of later admissions)
+  double density = 0.25;
   std::mutex lock;
 };
 
@@ -73,22 +78,48 @@ int n_types_of(int d, int m) { return n_types_rt(d, m); }
 // their slabs run through the single-mesh pipeline.
 bool batchable(const ftk2h_predicate_desc& p, int d) { return !(d == 4 && p.codimension == 2); }
 
-// Device bytes reserved for a stream: its two resident timesteps, the per-slab
-// work arrays of a batched pass (u64 mask + u32 offset per slab vertex) and an
-// output allowance of one node per 256 simplices (key, lambda, parent, track, one
-// edge each) over every slab it may append.  Actual use is reported by
-// ftk2h_server_stats; an allocation beyond the device's memory fails with E_OOM.
-uint64_t reservation(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int ncomp, uint64_t max_t) {
+// Bytes per node of a stream's payload on the device: key, lambda, union-find
+// parent and root, one edge.
+uint64_t bytes_per_node(const ftk2h_predicate_desc& p) {
+  const uint64_t esz = p.dtype == FTK2H_F64 ? 8 : 4;
+  return 8 + (uint64_t)(p.codimension + 1) * esz + 4 + 4 + 8;
+}
+
+// Payload of one slab at the server's measured density (nodes per slab vertex).
+uint64_t slab_payload(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, double density) {
+  uint64_t nsp = 1;
+  for (int j = 0; j < sd; ++j) nsp *= dims[j];
+  return (uint64_t)(density * 2.0 * (double)nsp * (double)bytes_per_node(p));
+}
+
+// Committed payload a stream keeps on the device before it retires it to host
+// memory: one slab, and at least 4 MiB, since a retirement synchronises the stream.
+uint64_t retire_threshold(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, double density) {
+  return std::max<uint64_t>(slab_payload(sd, dims, p, density), (uint64_t)4 << 20);
+}
+
+// Device bytes reserved for a stream, at the measured density:
+//   * its two resident timesteps and the work arrays of one slab extraction
+//     (u64 mask + u32 offset per slab vertex);
+//   * the open slab and the slab being extracted;
+//   * the committed arrays: their payload is at most the retirement threshold plus
+//     the slab that crosses it (and at most the whole stream's payload); their
+//     capacities double as they grow and start at kDevVecMinBytes each.
+// A denser request than any served before can exceed its reservation (the density
+// is then raised for later admissions); an allocation beyond the device's memory
+// fails with E_OOM.  ftk2h_server_stats reports the bytes actually held.
+uint64_t reservation(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int ncomp, uint64_t max_t,
+                     double density) {
   uint64_t nsp = 1;
   for (int j = 0; j < sd; ++j) nsp *= dims[j];
   const uint64_t esz = p.dtype == FTK2H_F64 ? 8 : 4;
-  const int d = sd + 1, m = p.codimension;
   const uint64_t fields = 2 * nsp * esz * (uint64_t)ncomp;
   const uint64_t work = 2 * nsp * 12;
-  const uint64_t per_node = 8 + (uint64_t)(m + 1) * esz + 4 + 4 + 8;
-  const uint64_t slabs = max_t > 1 ? max_t - 1 : 0;
-  const uint64_t out = slabs * nsp * (uint64_t)n_types_of(d, m) / 256 * per_node;
-  return fields + work + out;
+  if (max_t < 2) return fields;
+  const uint64_t slab = slab_payload(sd, dims, p, density);
+  const uint64_t committed =
+      std::min(retire_threshold(sd, dims, p, density) + slab, (max_t - 1) * slab);
+  return fields + work + 2 * slab + 2 * committed + 5 * (uint64_t)kDevVecMinBytes;
 }
 
 int finish_stream(ServedStream& S) {
@@ -115,6 +146,7 @@ int ftk2h_server_create(ftk2h_ctx* ctx, uint64_t hbm_budget_bytes, ftk2h_server*
   s->budget = hbm_budget_bytes;
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&s->st_batch, hipStreamNonBlocking));
   *out = s.release();
   return FTK2H_OK;
 }
@@ -125,12 +157,15 @@ int ftk2h_server_open(ftk2h_server* s, int spatial_ndims, const uint64_t* spatia
   if (spatial_ndims < 1 || spatial_ndims > 3) return fail(FTK2H_E_INVALID, "spatial dims must be 1..3");
   if (max_timesteps == 0) return fail(FTK2H_E_INVALID, "max_timesteps must be >= 1");
   std::lock_guard<std::mutex> g(s->lock);
-  const uint64_t need = reservation(spatial_ndims, spatial_dims, *pred, ncomp, max_timesteps);
+  const uint64_t need = reservation(spatial_ndims, spatial_dims, *pred, ncomp, max_timesteps, s->density);
   if (need > s->budget) return fail(FTK2H_E_OVERFLOW, "request exceeds the whole device-memory budget");
   if (s->reserved + need > s->budget) return fail(FTK2H_E_AGAIN, "device-memory budget in use; retry later");
   ServedStream S;
   int rc = ftk2h_stream_open(s->ctx, spatial_ndims, spatial_dims, pred, ncomp, &S.s);
   if (rc) return rc;
+  // finished nodes move to host memory past about one slab, so that the stream's
+  // device memory stays within its reservation however long it runs
+  ftk2h_stream_set_retire_bytes(S.s, retire_threshold(spatial_ndims, spatial_dims, *pred, s->density));
   S.pred = *pred;
   S.ncomp = ncomp;
   S.d = spatial_ndims + 1;
@@ -231,52 +266,82 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
     }
   }
   // the timestep copies were issued on each stream's own HIP stream
-  HIP_TRY(hipDeviceSynchronize());
-  // 2a. slabs that fill the device on their own: the single-mesh pipeline, one by one
-  for (Pending& p : solo) {
-    ftk2h_result* R = nullptr;
-    int rc = ftk2h_extract_regular(s->ctx, &p.job.mesh, &p.S->pred, p.views, p.S->ncomp, nullptr, nullptr, &R,
-                                   s->st);
-    if (!rc) rc = stream_take_slab(p.S->s, R);
-    if (rc) return rc;
-  }
-  // 2b. one batched pass per predicate template for the others
-  for (auto& [key, list] : groups) {
-    ftk2h_batch*& b = s->batches[key];
-    if (!b) {
-      const ServedStream& S0 = *list.front().S;
-      int rc = ftk2h_batch_create(s->ctx, &S0.pred, S0.d, &b);
-      if (rc) return rc;
-      ftk2h_batch_config cfg{4, 4, 0};  // persistent grid, dynamic chunks; shapes change every step
-      if ((rc = ftk2h_batch_set_config(b, &cfg))) return rc;
-    }
-    std::vector<ftk2h_job> jobs(list.size());
-    for (size_t i = 0; i < list.size(); ++i) {
-      jobs[i] = list[i].job;
-      jobs[i].comps = list[i].views;
-      jobs[i].scalar = nullptr;
-    }
-    std::vector<ftk2h_result*> res(list.size(), nullptr);
-    int rc = ftk2h_batch_run(b, (int)jobs.size(), jobs.data(), res.data(), nullptr);
-    if (rc) return rc;
-    // lower-plane node counts of every slab: one kernel each, one copy back
-    if ((rc = s->counts.ensure(list.size() * 4))) return rc;
-    if (s->host_counts.size() < list.size()) s->host_counts.resize(list.size());
-    uint32_t* dc = static_cast<uint32_t*>(s->counts.p);
-    for (size_t i = 0; i < list.size(); ++i) stream_launch_lower_count(list[i].S->s, res[i], dc + i, s->st);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(s->host_counts.data(), dc, list.size() * 4, hipMemcpyDeviceToHost, s->st));
-    HIP_TRY(hipStreamSynchronize(s->st));
-    for (size_t i = 0; i < list.size(); ++i) {
-      rc = stream_take_slab_n(list[i].S->s, res[i], s->host_counts[i]);
-      res[i] = nullptr;
-      if (rc) {
-        for (auto* r : res)
-          if (r) ftk2h_result_free(r);
-        return rc;
+  for (Pending& p : solo) HIP_TRY(hipStreamSynchronize(stream_hip(p.S->s)));
+  for (auto& [key, list] : groups)
+    for (Pending& p : list) HIP_TRY(hipStreamSynchronize(stream_hip(p.S->s)));
+  // 2b (on a worker thread and its own HIP stream): one batched pass per predicate
+  // template for the small slabs, while 2a runs the slabs that fill the device
+  int batch_rc = FTK2H_OK;
+  std::string batch_err;
+  double batch_density = 0.0;
+  auto run_groups = [&]() {
+    int rc = FTK2H_OK;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) rc = fail(FTK2H_E_HIP, "hipSetDevice");
+    for (auto& [key, list] : groups) {
+      if (rc) break;
+      ftk2h_batch*& b = s->batches[key];
+      if (!b) {
+        const ServedStream& S0 = *list.front().S;
+        rc = ftk2h_batch_create(s->ctx, &S0.pred, S0.d, &b);
+        if (rc) break;
+        ftk2h_batch_config cfg{4, 4, 0};  // persistent grid, dynamic chunks; shapes change every step
+        if ((rc = ftk2h_batch_set_config(b, &cfg))) break;
+      }
+      std::vector<ftk2h_job> jobs(list.size());
+      for (size_t i = 0; i < list.size(); ++i) {
+        jobs[i] = list[i].job;
+        jobs[i].comps = list[i].views;
+        jobs[i].scalar = nullptr;
+      }
+      std::vector<ftk2h_result*> res(list.size(), nullptr);
+      rc = ftk2h_batch_run(b, (int)jobs.size(), jobs.data(), res.data(), s->st_batch);
+      if (rc) break;
+      // lower-plane node counts of every slab: one kernel each, one copy back
+      if ((rc = s->counts.ensure(list.size() * 4))) break;
+      if (s->host_counts.size() < list.size()) s->host_counts.resize(list.size());
+      uint32_t* dc = static_cast<uint32_t*>(s->counts.p);
+      for (size_t i = 0; i < list.size(); ++i) stream_launch_lower_count(list[i].S->s, res[i], dc + i, s->st_batch);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(s->host_counts.data(), dc, list.size() * 4, hipMemcpyDeviceToHost, s->st_batch);
+      if (e == hipSuccess) e = hipStreamSynchronize(s->st_batch);
+      if (e != hipSuccess) {
+        rc = fail(FTK2H_E_HIP, std::string("server counts: ") + hipGetErrorString(e));
+        for (auto* r : res) ftk2h_result_free(r);
+        break;
+      }
+      for (size_t i = 0; i < list.size(); ++i) {
+        uint64_t sv = 1;
+        for (int j = 0; j < list[i].job.mesh.ndims; ++j) sv *= list[i].job.mesh.local_dims[j];
+        batch_density = std::max(batch_density, (double)res[i]->n_nodes / (double)sv);
+        if (!rc) rc = stream_take_slab_n(list[i].S->s, res[i], s->host_counts[i]);
+        else ftk2h_result_free(res[i]);
+        res[i] = nullptr;
       }
     }
+    batch_rc = rc;
+    if (rc) batch_err = error_slot();
+  };
+  std::thread worker;
+  if (!groups.empty()) worker = std::thread(run_groups);
+  // 2a. slabs that fill the device on their own: the single-mesh pipeline, one by one
+  int solo_rc = FTK2H_OK;
+  for (Pending& p : solo) {
+    ftk2h_result* R = nullptr;
+    solo_rc = ftk2h_extract_regular(s->ctx, &p.job.mesh, &p.S->pred, p.views, p.S->ncomp, nullptr, nullptr, &R,
+                                    s->st);
+    if (solo_rc) break;
+    uint64_t sv = 1;
+    for (int j = 0; j < p.job.mesh.ndims; ++j) sv *= p.job.mesh.local_dims[j];
+    s->density = std::max(s->density, (double)R->n_nodes / (double)sv);
+    solo_rc = stream_take_slab(p.S->s, R);
+    if (solo_rc) break;
   }
+  const std::string solo_err = solo_rc ? error_slot() : std::string();
+  if (worker.joinable()) worker.join();
+  s->density = std::max(s->density, batch_density);
+  if (solo_rc) return fail(solo_rc, solo_err);
+  if (batch_rc) return fail(batch_rc, batch_err);
   // 3. closed and drained streams leave with their result
   for (auto& [id, S] : s->streams) {
     if (S.finished || !S.closed || !S.queue.empty() || stream_slab_ready(S.s)) continue;
@@ -330,6 +395,7 @@ void ftk2h_server_destroy(ftk2h_server* s) {
   for (auto& [k, b] : s->batches) ftk2h_batch_destroy(b);
   s->counts.release();
   if (s->st) (void)hipStreamDestroy(s->st);
+  if (s->st_batch) (void)hipStreamDestroy(s->st_batch);
   delete s;
 }
 

@@ -21,13 +21,14 @@ using namespace ftk2h_internal;
 
 namespace {
 
-// Grow-to-fit device array that keeps its contents.
+// Grow-to-fit device array that keeps its contents: capacity starts at
+// kDevVecMinBytes and doubles, so it stays below twice the largest size asked for.
 struct DevVec {
   void* p = nullptr;
   size_t bytes = 0, cap = 0;
   int reserve(size_t n, hipStream_t st) {
     if (n <= cap) return FTK2H_OK;
-    size_t nc = cap ? cap : (size_t)1 << 20;
+    size_t nc = cap ? cap : kDevVecMinBytes;
     while (nc < n) nc *= 2;
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, nc);
@@ -388,6 +389,8 @@ int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow) {
 }
 
 uint64_t stream_timesteps(const ftk2h_stream* S) { return S->t; }
+
+hipStream_t stream_hip(const ftk2h_stream* S) { return S->st; }
 
 uint64_t stream_device_bytes(const ftk2h_stream* S) {
   uint64_t b = 2 * S->nsp * S->esz * (uint64_t)S->ncomp;

@@ -5,6 +5,10 @@
 
 namespace ftk2h_internal {
 
+// Smallest capacity of each of a stream's five committed-payload device arrays
+// (keys, lambda, union-find parents, cells, roots).
+constexpr size_t kDevVecMinBytes = (size_t)64 << 10;
+
 // Copy the next timestep into its device half.  Fails if a slab is pending.
 int stream_put(ftk2h_stream* S, const ftk2h_field_view* comps);
 // A slab [t-2, t-1] awaits extraction.
@@ -18,6 +22,8 @@ int stream_take_slab(ftk2h_stream* S, ftk2h_result* R);
 int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow);
 void stream_launch_lower_count(const ftk2h_stream* S, const ftk2h_result* R, uint32_t* out, hipStream_t st);
 uint64_t stream_timesteps(const ftk2h_stream* S);
+// The HIP stream a stream's timestep copies and re-indexing run on.
+hipStream_t stream_hip(const ftk2h_stream* S);
 // Device memory held by the stream (timestep halves + committed nodes/cells).
 uint64_t stream_device_bytes(const ftk2h_stream* S);
 
