@@ -96,6 +96,7 @@ SIGNATURES = {
     "ftk2h_stream_finish": (C.c_int, [_P, C.POINTER(_P)]),
     "ftk2h_stream_close": (None, [_P]),
     "ftk2h_stream_set_retire_bytes": (C.c_int, [_P, C.c_uint64]),
+    "ftk2h_stream_set_window": (C.c_int, [_P, C.c_int]),
     "ftk2h_stream_resident_bytes": (C.c_uint64, [_P]),
     "ftk2h_stream_retired_nodes": (C.c_uint64, [_P]),
     "ftk2h_batch_create": (C.c_int, [_P, C.POINTER(PredicateDesc), C.c_int, C.POINTER(_P)]),

@@ -26,6 +26,11 @@ namespace {
 // kernels are tuned for one large mesh, instead of the batched pass.
 constexpr uint64_t SOLO_SLAB_VERTICES = (uint64_t)1 << 21;
 
+// A stream extracts the timesteps it has queued in windows of up to this many
+// vertices (2^27 = 256^3 x 8): the planes inside a window are classified once
+// instead of once per slab, and one launch covers several slabs.
+constexpr uint64_t WINDOW_VERTICES = (uint64_t)1 << 27;
+
 struct QueuedStep {
   ftk2h_field_view views[3];
   std::vector<char> host[3];  // owned copies of host views
@@ -63,8 +68,8 @@ struct ftk2h_server {
   std::vector<uint32_t> host_counts;
   hipStream_t st = nullptr;         // single-mesh slabs
   hipStream_t st_batch = nullptr;   // batched passes, concurrent with them
-  // nodes per slab vertex, the largest seen in a served slab so far (sizes the
-  // reservations of later admissions)
+  // nodes per window vertex, the largest seen in a served window so far (sizes
+  // the reservations of later admissions)
   double density = 0.25;
   std::mutex lock;
 };
@@ -85,41 +90,53 @@ uint64_t bytes_per_node(const ftk2h_predicate_desc& p) {
   return 8 + (uint64_t)(p.codimension + 1) * esz + 4 + 4 + 8;
 }
 
-// Payload of one slab at the server's measured density (nodes per slab vertex).
-uint64_t slab_payload(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, double density) {
+uint64_t spatial_vertices(int sd, const uint64_t* dims) {
   uint64_t nsp = 1;
   for (int j = 0; j < sd; ++j) nsp *= dims[j];
-  return (uint64_t)(density * 2.0 * (double)nsp * (double)bytes_per_node(p));
+  return nsp;
+}
+
+// Timesteps of a stream's largest window: up to WINDOW_VERTICES vertices, at least
+// one slab, at most the whole stream.
+int window_planes(int sd, const uint64_t* dims, uint64_t max_t) {
+  const uint64_t nsp = spatial_vertices(sd, dims);
+  const uint64_t fit = std::max<uint64_t>(2, WINDOW_VERTICES / nsp);
+  return (int)std::max<uint64_t>(2, std::min<uint64_t>(fit, std::min<uint64_t>(max_t, 64)));
+}
+
+// Payload of one window's result at the server's measured density (nodes per
+// window vertex).
+uint64_t window_payload(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int planes, double density) {
+  return (uint64_t)(density * (double)planes * (double)spatial_vertices(sd, dims) * (double)bytes_per_node(p));
 }
 
 // Committed payload a stream keeps on the device before it retires it to host
-// memory: one slab, and at least 4 MiB, since a retirement synchronises the stream.
-uint64_t retire_threshold(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, double density) {
-  return std::max<uint64_t>(slab_payload(sd, dims, p, density), (uint64_t)4 << 20);
+// memory: one window, and at least 4 MiB, since a retirement synchronises the stream.
+uint64_t retire_threshold(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int planes, double density) {
+  return std::max<uint64_t>(window_payload(sd, dims, p, planes, density), (uint64_t)4 << 20);
 }
 
 // Device bytes reserved for a stream, at the measured density:
-//   * its two resident timesteps and the work arrays of one slab extraction
-//     (u64 mask + u32 offset per slab vertex);
-//   * the open slab and the slab being extracted;
+//   * its window's timesteps and the work arrays of one window's extraction
+//     (mask, pending and degenerate masks, node offset, cell count: 30 B per vertex);
+//   * the open window and the window being extracted;
 //   * the committed arrays: their payload is at most the retirement threshold plus
-//     the slab that crosses it (and at most the whole stream's payload); their
+//     the window that crosses it (and at most the whole stream's payload); their
 //     capacities double as they grow and start at kDevVecMinBytes each.
 // A denser request than any served before can exceed its reservation (the density
 // is then raised for later admissions); an allocation beyond the device's memory
 // fails with E_OOM.  ftk2h_server_stats reports the bytes actually held.
 uint64_t reservation(int sd, const uint64_t* dims, const ftk2h_predicate_desc& p, int ncomp, uint64_t max_t,
-                     double density) {
-  uint64_t nsp = 1;
-  for (int j = 0; j < sd; ++j) nsp *= dims[j];
+                     int planes, double density) {
+  const uint64_t nsp = spatial_vertices(sd, dims);
   const uint64_t esz = p.dtype == FTK2H_F64 ? 8 : 4;
-  const uint64_t fields = 2 * nsp * esz * (uint64_t)ncomp;
-  const uint64_t work = 2 * nsp * 12;
+  const uint64_t fields = (uint64_t)planes * nsp * esz * (uint64_t)ncomp;
   if (max_t < 2) return fields;
-  const uint64_t slab = slab_payload(sd, dims, p, density);
-  const uint64_t committed =
-      std::min(retire_threshold(sd, dims, p, density) + slab, (max_t - 1) * slab);
-  return fields + work + 2 * slab + 2 * committed + 5 * (uint64_t)kDevVecMinBytes;
+  const uint64_t work = (uint64_t)planes * nsp * 30;
+  const uint64_t win = window_payload(sd, dims, p, planes, density);
+  const uint64_t whole = (uint64_t)(density * (double)max_t * (double)nsp * (double)bytes_per_node(p));
+  const uint64_t committed = std::min(retire_threshold(sd, dims, p, planes, density) + win, whole);
+  return fields + work + 2 * win + 2 * committed + 5 * (uint64_t)kDevVecMinBytes;
 }
 
 int finish_stream(ServedStream& S) {
@@ -157,15 +174,25 @@ int ftk2h_server_open(ftk2h_server* s, int spatial_ndims, const uint64_t* spatia
   if (spatial_ndims < 1 || spatial_ndims > 3) return fail(FTK2H_E_INVALID, "spatial dims must be 1..3");
   if (max_timesteps == 0) return fail(FTK2H_E_INVALID, "max_timesteps must be >= 1");
   std::lock_guard<std::mutex> g(s->lock);
-  const uint64_t need = reservation(spatial_ndims, spatial_dims, *pred, ncomp, max_timesteps, s->density);
-  if (need > s->budget) return fail(FTK2H_E_OVERFLOW, "request exceeds the whole device-memory budget");
+  // the largest window whose reservation fits what is left of the budget
+  int planes = window_planes(spatial_ndims, spatial_dims, max_timesteps);
+  auto need_of = [&](int w) {
+    return reservation(spatial_ndims, spatial_dims, *pred, ncomp, max_timesteps, w, s->density);
+  };
+  uint64_t need = need_of(planes);
+  while (planes > 2 && s->reserved + need > s->budget) need = need_of(--planes);
+  if (need_of(2) > s->budget) return fail(FTK2H_E_OVERFLOW, "request exceeds the whole device-memory budget");
   if (s->reserved + need > s->budget) return fail(FTK2H_E_AGAIN, "device-memory budget in use; retry later");
   ServedStream S;
   int rc = ftk2h_stream_open(s->ctx, spatial_ndims, spatial_dims, pred, ncomp, &S.s);
   if (rc) return rc;
-  // finished nodes move to host memory past about one slab, so that the stream's
+  if ((rc = stream_set_window(S.s, planes))) {
+    ftk2h_stream_close(S.s);
+    return rc;
+  }
+  // finished nodes move to host memory past about one window, so that the stream's
   // device memory stays within its reservation however long it runs
-  ftk2h_stream_set_retire_bytes(S.s, retire_threshold(spatial_ndims, spatial_dims, *pred, s->density));
+  ftk2h_stream_set_retire_bytes(S.s, retire_threshold(spatial_ndims, spatial_dims, *pred, planes, s->density));
   S.pred = *pred;
   S.ncomp = ncomp;
   S.d = spatial_ndims + 1;
@@ -247,12 +274,14 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
   std::vector<Pending> solo;
   for (auto& [id, S] : s->streams) {
     if (S.finished) continue;
+    // the queued timesteps fill the stream's window (a partial one when fewer are
+    // queued: it is extracted now rather than waiting for more)
     while (!S.queue.empty() && !stream_slab_ready(S.s)) {
       int rc = stream_put(S.s, S.queue.front().views);
       if (rc) return rc;
       S.queue.pop_front();
     }
-    if (stream_slab_ready(S.s)) {
+    if (stream_window_pending(S.s)) {
       Pending p;
       p.S = &S;
       std::memset(&p.job, 0, sizeof(p.job));
@@ -344,7 +373,7 @@ int ftk2h_server_step(ftk2h_server* s, uint64_t* done, int max_done, int* n_done
   if (batch_rc) return fail(batch_rc, batch_err);
   // 3. closed and drained streams leave with their result
   for (auto& [id, S] : s->streams) {
-    if (S.finished || !S.closed || !S.queue.empty() || stream_slab_ready(S.s)) continue;
+    if (S.finished || !S.closed || !S.queue.empty() || stream_window_pending(S.s)) continue;
     int rc = finish_stream(S);
     if (rc) return rc;
     if (*n_done < max_done) done[*n_done] = id;

@@ -1,12 +1,18 @@
 // Streaming extraction (SimplicialEngine::execute_stream, hguo/ftk2
 // include/ftk2/core/engine.hpp:214-270): timesteps arrive one at a time; each new
 // timestep t closes the slab [t-1, t], a RegularSimplicialMesh with local extent 2 in
-// time and offset t-1 (engine.hpp:261-266).  Device-resident state:
-//   * the last timestep of every field component (ping-pong halves of one buffer,
-//     so a slab is a plain strided view);
-//   * the previous slab's result: its upper-plane nodes (pure spatial simplices at
-//     time t) are also nodes of the next slab, so that slab stays open until the
-//     next one arrives and its edges can be re-indexed by key;
+// time and offset t-1 (engine.hpp:261-266).  A stream may also extract a window of
+// several slabs [t0, t0 + W] at once (ftk2h_stream_set_window; the server does so
+// with the timesteps it has queued): the same mesh with local extent W + 1, so the
+// planes inside the window are classified once instead of twice.  Device-resident
+// state:
+//   * the timesteps of the current window in `wcap` slots per field component; a
+//     window runs over consecutive slots, up or down, and the next one starts at
+//     its last slot going the other way, so a window is a plain strided view (for
+//     wcap = 2, the two halves of a ping-pong buffer);
+//   * the previous window's result: its upper-plane nodes (pure spatial simplices
+//     at its last time) are also nodes of the next window, so that window stays
+//     open until the next one arrives and its edges can be re-indexed by key;
 //   * the committed nodes/edges and a union-find over them (the open frontier of
 //     the track components).  Once the committed payload on the device exceeds the
 //     retirement threshold, finished nodes, lambda and cells move to host memory and
@@ -124,9 +130,13 @@ struct ftk2h_stream {
   int ntypes = 0;
   int nv = 0;  // indices per cell (0: no connectivity)
   hipStream_t st = nullptr;
-  void* fields[3] = {nullptr, nullptr, nullptr};  // 2 * nsp values each (ping-pong)
+  void* fields[3] = {nullptr, nullptr, nullptr};  // wcap * nsp values each
+  int wcap = 2;                                   // planes of a full window (slabs + 1)
+  int64_t s0 = 0;                                 // slot of the window's first plane
+  int dir = 1;                                    // slot step from one plane to the next
+  int np = 0;                                     // planes of the window appended so far
   uint64_t t = 0;                                 // timesteps appended
-  uint64_t slabs = 0;                             // slabs extracted
+  uint64_t slabs = 0;                             // windows extracted
   ftk2h_result* open = nullptr;                   // previous slab, not committed yet
   uint32_t open_nlow = 0;
   uint32_t committed = 0;                         // committed nodes
@@ -151,9 +161,8 @@ struct ftk2h_stream {
 
 namespace {
 
-int slab_lower_count(ftk2h_stream* S, const ftk2h_result* R, uint64_t slab_t, uint32_t* out) {
-  const uint64_t bound = (slab_t + 1) * S->nsp * (uint64_t)S->ntypes;
-  k_lower_bound<<<1, 1, 0, S->st>>>(R->keys, R->n_nodes, bound, S->scratch);
+int slab_lower_count(ftk2h_stream* S, const ftk2h_result* R, uint32_t* out) {
+  k_lower_bound<<<1, 1, 0, S->st>>>(R->keys, R->n_nodes, stream_lower_bound_key(S), S->scratch);
   HIP_TRY(hipMemcpyAsync(out, S->scratch, 4, hipMemcpyDeviceToHost, S->st));
   HIP_TRY(hipStreamSynchronize(S->st));
   return FTK2H_OK;
@@ -280,11 +289,11 @@ int retire(ftk2h_stream* S) {
 namespace ftk2h_internal {
 
 int stream_put(ftk2h_stream* S, const ftk2h_field_view* comps) {
-  if (stream_slab_ready(S)) return fail(FTK2H_E_INVALID, "stream: the pending slab must be extracted first");
-  const uint64_t half = S->t % 2;
-  // copy the timestep into its ping-pong half (x fastest, contiguous)
+  if (stream_slab_ready(S)) return fail(FTK2H_E_INVALID, "stream: the pending window must be extracted first");
+  const int64_t slot = S->s0 + (int64_t)S->dir * S->np;
+  // copy the timestep into its slot (x fastest, contiguous)
   for (int c = 0; c < S->ncomp; ++c) {
-    char* dst = static_cast<char*>(S->fields[c]) + half * S->nsp * S->esz;
+    char* dst = static_cast<char*>(S->fields[c]) + slot * S->nsp * S->esz;
     const ftk2h_field_view& v = comps[c];
     if (!v.ptr) return fail(FTK2H_E_INVALID, "null field pointer");
     int64_t want = 1;
@@ -314,17 +323,36 @@ int stream_put(ftk2h_stream* S, const ftk2h_field_view* comps) {
       HIP_TRY(hipStreamSynchronize(S->st));
     }
   }
+  S->np++;
   S->t++;
   return FTK2H_OK;
 }
 
-bool stream_slab_ready(const ftk2h_stream* S) { return S->t >= 2 && S->slabs + 1 < S->t; }
+bool stream_slab_ready(const ftk2h_stream* S) { return S->np >= S->wcap; }
 
-// Slab [t-2, t-1]: views over both halves (time stride = distance between halves),
-// offset t-2 in time, global time extent t.
+bool stream_window_pending(const ftk2h_stream* S) { return S->np >= 2; }
+
+int stream_set_window(ftk2h_stream* S, int planes) {
+  if (planes < 2) return fail(FTK2H_E_INVALID, "stream: a window has at least 2 timesteps");
+  if (S->t) return fail(FTK2H_E_INVALID, "stream: the window is set before the first timestep");
+  if (planes == S->wcap) return FTK2H_OK;
+  HIP_TRY(hipSetDevice(S->ctx->device));
+  for (int c = 0; c < S->ncomp; ++c) {
+    if (S->fields[c]) (void)hipFree(S->fields[c]);
+    S->fields[c] = nullptr;
+  }
+  S->wcap = planes;
+  for (int c = 0; c < S->ncomp; ++c) {
+    hipError_t e = hipMalloc(&S->fields[c], (size_t)planes * S->nsp * S->esz);
+    if (e != hipSuccess) return fail(FTK2H_E_OOM, std::string("stream window: ") + hipGetErrorString(e));
+  }
+  return FTK2H_OK;
+}
+
+// The window [t - np, t - 1]: views over its slots (time stride = +-nsp), offset
+// t - np in time, global time extent t.
 void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_view* views) {
-  const uint64_t slab_t = S->t - 2;
-  const uint64_t hp = slab_t % 2;
+  const uint64_t slab_t = S->t - (uint64_t)S->np;
   std::memset(md, 0, sizeof(*md));
   md->ndims = S->sd + 1;
   for (int j = 0; j < 4; ++j) {
@@ -332,19 +360,19 @@ void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_vi
     md->offset[j] = 0;
     md->global_dims[j] = j < S->sd ? S->spatial[j] : 1;
   }
-  md->local_dims[S->sd] = 2;
+  md->local_dims[S->sd] = (uint64_t)S->np;
   md->offset[S->sd] = slab_t;
-  md->global_dims[S->sd] = slab_t + 2;
+  md->global_dims[S->sd] = S->t;
   for (int c = 0; c < S->ncomp; ++c) {
     views[c] = ftk2h_field_view{};
     views[c].on_device = 1;
-    views[c].ptr = static_cast<char*>(S->fields[c]) + hp * S->nsp * S->esz;
+    views[c].ptr = static_cast<char*>(S->fields[c]) + S->s0 * S->nsp * S->esz;
     int64_t s = 1;
     for (int j = 0; j < S->sd; ++j) {
       views[c].strides[j] = s;
       s *= (int64_t)S->spatial[j];
     }
-    views[c].strides[S->sd] = hp == 0 ? (int64_t)S->nsp : -(int64_t)S->nsp;
+    views[c].strides[S->sd] = S->dir * (int64_t)S->nsp;
   }
 }
 
@@ -352,7 +380,7 @@ void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_vi
 // previously open slab against it and keep R open.
 int stream_take_slab(ftk2h_stream* S, ftk2h_result* R) {
   uint32_t nlow = 0;
-  int rc = slab_lower_count(S, R, S->t - 2, &nlow);
+  int rc = slab_lower_count(S, R, &nlow);
   if (rc) {
     ftk2h_result_free(R);
     return rc;
@@ -385,6 +413,22 @@ int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow) {
   S->open = R;
   S->open_nlow = nlow;
   S->slabs++;
+  // The next window starts at this one's last plane and runs the other way.  After
+  // a partial window fewer than wcap slots lie that way: the plane then moves to
+  // slot 0 first (the extraction that read the slots has completed).
+  const int64_t last = S->s0 + (int64_t)S->dir * (S->np - 1);
+  S->dir = -S->dir;
+  S->s0 = last;
+  S->np = 1;
+  const int64_t room = S->dir > 0 ? S->wcap - last : last + 1;
+  if (room < S->wcap) {
+    for (int c = 0; c < S->ncomp; ++c) {
+      char* f = static_cast<char*>(S->fields[c]);
+      HIP_TRY(hipMemcpyAsync(f, f + last * S->nsp * S->esz, S->nsp * S->esz, hipMemcpyDeviceToDevice, S->st));
+    }
+    S->s0 = 0;
+    S->dir = 1;
+  }
   return FTK2H_OK;
 }
 
@@ -393,7 +437,7 @@ uint64_t stream_timesteps(const ftk2h_stream* S) { return S->t; }
 hipStream_t stream_hip(const ftk2h_stream* S) { return S->st; }
 
 uint64_t stream_device_bytes(const ftk2h_stream* S) {
-  uint64_t b = 2 * S->nsp * S->esz * (uint64_t)S->ncomp;
+  uint64_t b = (uint64_t)S->wcap * S->nsp * S->esz * (uint64_t)S->ncomp;
   b += S->keys.cap + S->lambda.cap + S->parent.cap + S->cells.cap + S->roots.cap;
   return b;
 }
@@ -430,7 +474,7 @@ int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial
   else S->nv = k == 1 ? 2 : (k == 2 ? 3 : 0);
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
-  for (int c = 0; c < ncomp; ++c) HIP_TRY(hipMalloc(&S->fields[c], 2 * S->nsp * S->esz));
+  for (int c = 0; c < ncomp; ++c) HIP_TRY(hipMalloc(&S->fields[c], (size_t)S->wcap * S->nsp * S->esz));
   HIP_TRY(hipEventCreateWithFlags(&S->retiring_ev, hipEventDisableTiming));
   // FTK2H_STREAM_NODE_LIMIT lowers the committed-node limit (tests drive the overflow check)
   if (const char* lim = std::getenv("FTK2H_STREAM_NODE_LIMIT")) {
@@ -443,6 +487,17 @@ int ftk2h_stream_open(ftk2h_ctx* ctx, int spatial_ndims, const uint64_t* spatial
   return FTK2H_OK;
 }
 
+// Extracts the pending window on the stream's own HIP stream and takes it.
+static int extract_window(ftk2h_stream* S) {
+  ftk2h_mesh_desc md;
+  ftk2h_field_view views[3];
+  stream_slab_desc(S, &md, views);
+  ftk2h_result* R = nullptr;
+  int rc = ftk2h_extract_regular(S->ctx, &md, &S->pred, views, S->ncomp, nullptr, nullptr, &R, S->st);
+  if (!rc) rc = stream_take_slab(S, R);
+  return rc;
+}
+
 int ftk2h_stream_append(ftk2h_stream* S, const ftk2h_field_view* comps, void* hip_stream) {
   if (!S || !comps) return fail(FTK2H_E_INVALID, "null argument");
   (void)hip_stream;  // the stream works on its own HIP stream
@@ -452,14 +507,7 @@ int ftk2h_stream_append(ftk2h_stream* S, const ftk2h_field_view* comps, void* hi
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, S->st));
   int rc = stream_put(S, comps);
-  if (!rc && stream_slab_ready(S)) {
-    ftk2h_mesh_desc md;
-    ftk2h_field_view views[3];
-    stream_slab_desc(S, &md, views);
-    ftk2h_result* R = nullptr;
-    rc = ftk2h_extract_regular(S->ctx, &md, &S->pred, views, S->ncomp, nullptr, nullptr, &R, S->st);
-    if (!rc) rc = stream_take_slab(S, R);
-  }
+  if (!rc && stream_slab_ready(S)) rc = extract_window(S);
   if (!rc) {
     HIP_TRY(hipEventRecord(e1, S->st));
     HIP_TRY(hipEventSynchronize(e1));
@@ -477,6 +525,8 @@ int ftk2h_stream_finish(ftk2h_stream* S, ftk2h_result** out) {
   *out = nullptr;
   HIP_TRY(hipSetDevice(S->ctx->device));
   int rc;
+  // a partial window (windows of more than one slab)
+  if (stream_window_pending(S) && (rc = extract_window(S))) return rc;
   if (S->open) {
     rc = commit_open(S, nullptr, 0);
     const int rc2 = retire_result(S, S->open);
@@ -570,6 +620,11 @@ void ftk2h_stream_close(ftk2h_stream* S) {
   delete S;
 }
 
+int ftk2h_stream_set_window(ftk2h_stream* S, int timesteps) {
+  if (!S) return fail(FTK2H_E_INVALID, "null argument");
+  return stream_set_window(S, timesteps);
+}
+
 int ftk2h_stream_set_retire_bytes(ftk2h_stream* S, uint64_t bytes) {
   if (!S) return fail(FTK2H_E_INVALID, "null argument");
   S->retire_bytes = bytes;
@@ -578,7 +633,7 @@ int ftk2h_stream_set_retire_bytes(ftk2h_stream* S, uint64_t bytes) {
 
 uint64_t ftk2h_stream_resident_bytes(const ftk2h_stream* S) {
   if (!S) return 0;
-  return 2 * S->nsp * S->esz * (uint64_t)S->ncomp + S->keys.bytes + S->lambda.bytes + S->parent.bytes +
+  return (uint64_t)S->wcap * S->nsp * S->esz * (uint64_t)S->ncomp + S->keys.bytes + S->lambda.bytes + S->parent.bytes +
          S->cells.bytes;
 }
 

@@ -410,10 +410,12 @@ class Stream:
     """Timestep-by-timestep extraction (ftk2h_stream_*): execute_stream semantics with
     the last timestep and the open track frontier resident on the GPU.  retire_bytes:
     device payload of finished nodes/lambda/cells beyond which they move to host
-    memory (0: after every slab; None: the library default, 8 GiB)."""
+    memory (0: after every slab; None: the library default, 8 GiB).  window: timesteps
+    per extraction (2: one slab per appended timestep, as the reference; W + 1: W
+    slabs at once, the last partial window in finish())."""
 
     def __init__(self, spatial_dims, kind: int, m: int, dtype: int, ncomp: int, thresholds=(0.0, 0.0),
-                 sos_q: float = 1e6, device: int = 0, retire_bytes=None):
+                 sos_q: float = 1e6, device: int = 0, retire_bytes=None, window: int = 2):
         self.spatial = [int(x) for x in spatial_dims]
         pd = _lib.PredicateDesc()
         pd.kind = kind
@@ -430,6 +432,8 @@ class Stream:
         self.ncomp = ncomp
         if retire_bytes is not None:
             _lib.check(_lib.lib().ftk2h_stream_set_retire_bytes(self.h, int(retire_bytes)))
+        if window != 2:
+            _lib.check(_lib.lib().ftk2h_stream_set_window(self.h, int(window)))
 
     def resident_bytes(self) -> int:
         """Device bytes in use (timestep buffers + committed state still on the GPU)."""

@@ -237,7 +237,15 @@ void ftk2h_stream_close(ftk2h_stream* s);
  * Retirement copies through pageable host memory and waits for the copy, so a low
  * threshold trades append time for device memory. */
 int ftk2h_stream_set_retire_bytes(ftk2h_stream* s, uint64_t bytes);
-/* Device bytes in use by the stream: both timestep buffers and the committed
+/* Windows of several slabs.  No counterpart in the reference, whose streaming path
+ * extracts one slab [t-1, t] per timestep (engine.hpp:214-270).  With `timesteps` =
+ * W + 1 > 2 the stream keeps W + 1 timesteps on the device and extracts a window of
+ * W slabs once it is full (and the last, partial one in ftk2h_stream_finish): one
+ * mesh of local time extent W + 1, so every plane inside a window is classified
+ * once instead of twice.  The result is the same.  Set before the first append;
+ * default 2 (one slab per timestep, as the reference). */
+int ftk2h_stream_set_window(ftk2h_stream* s, int timesteps);
+/* Device bytes in use by the stream: its window's timestep buffers and the committed
  * nodes, lambda, cells and union-find parents still on the device (allocations
  * grow to at most about the retirement threshold plus one slab). */
 uint64_t ftk2h_stream_resident_bytes(const ftk2h_stream* s);

@@ -236,15 +236,16 @@ def test_several_device_filling_slabs_in_one_step():
 
 
 def test_noise_heavy_stream_stays_within_its_reservation():
-    """A long, node-dense 3-D stream: served streams retire finished nodes to host
-    memory past the retirement threshold, so once the server has measured the density
-    (one short request of the same kind first), the device memory a stream holds stays
-    within what admission reserved, well below its whole payload; the result is
-    unchanged."""
+    """A long, node-dense 3-D stream under a tight budget: the server admits it with
+    the largest window of timesteps that fits, and served streams retire finished
+    nodes to host memory past the retirement threshold, so once the server has
+    measured the density (one short request of the same kind first), the device
+    memory a stream holds stays within what admission reserved and below its whole
+    payload; the result is unchanged."""
     T = 20
     comps = synth.cp3d_config(n=(34, 32, 30), nt=T, ngauss=6, seed=81, noise=0.05)
     steps = [[c[..., t] for c in comps] for t in range(T)]
-    srv = engine.Server(160 << 20)
+    srv = engine.Server(48 << 20)
     # learn the density from a two-timestep request
     warm = srv.open((34, 32, 30), _lib.PRED_CRITICAL_POINT, 3, _lib.F64, 3, 2)
     for c in steps[:2]:
@@ -273,6 +274,6 @@ def test_noise_heavy_stream_stays_within_its_reservation():
     payload = len(got.vertices) * (8 + 4 * 8 + 4 + 4 + 8)
     say(f"[serve] dense stream: {len(got.vertices)} nodes ({payload} B), device peak {peak} B, reserved {reserved} B")
     assert peak <= reserved  # the device holds no more than admission reserved...
-    assert 2 * peak < payload  # ...and much less than the whole payload
+    assert peak < payload  # ...and never the whole payload
     rq = serve.Request(0, 0.0, (34, 32, 30), _lib.PRED_CRITICAL_POINT, 3, _lib.F64, steps)
     same(got, alone(rq), "dense stream")

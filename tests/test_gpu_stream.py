@@ -10,10 +10,10 @@ from ftk2_amd import _lib, engine, synth
 pytestmark = pytest.mark.gpu
 
 
-def run_stream(comps, kind, m, thr=(0.0, 0.0)):
+def run_stream(comps, kind, m, thr=(0.0, 0.0), window=2, retire=None):
     nt = comps[0].shape[-1]
     dt = _lib.F64 if comps[0].dtype == np.float64 else _lib.F32
-    s = engine.Stream(comps[0].shape[:-1], kind, m, dt, len(comps), thr)
+    s = engine.Stream(comps[0].shape[:-1], kind, m, dt, len(comps), thr, window=window, retire_bytes=retire)
     for t in range(nt):
         s.append([np.asfortranarray(c[..., t]) for c in comps])
     r = s.finish().to_complex()
@@ -135,6 +135,47 @@ def test_stream_retirement_equals_oneshot(case, retire):
     assert resident[-1] < res_keep[-1]
     if retire == 0:
         assert max(resident[2:]) < res_keep[-1]
+
+
+def window_case(case):
+    if case == "cp2d":
+        return list(synth.cp2d_config(nx=40, ny=36, nt=9, seed=21)), 0, 2, (0, 0)
+    if case == "cp3d":
+        return synth.cp3d_config(n=14, nt=7, ngauss=6, seed=22, noise=0.02), 0, 3, (0, 0)
+    if case == "contour":
+        return [synth.scalar_config(n=16, nt=8, ngauss=8, seed=23, dtype=np.float64)], 1, 1, (0.0, 0)
+    comps = [synth.scalar_config(n=14, nt=7, ngauss=8, seed=s, dtype=np.float64) for s in (24, 25)]
+    return comps, 2, 2, (0.0, 0.0)
+
+
+@pytest.mark.parametrize("case", ["cp2d", "cp3d", "contour", "fiber"])
+def test_stream_windows_equal_oneshot(case):
+    """Windows of several slabs (ftk2h_stream_set_window): full windows, a last
+    partial window, one window over the whole stream, and a window larger than the
+    stream all give the one-shot complex bit for bit, as the slab-by-slab stream does;
+    with retirement after every window too."""
+    print(f"[stream-window] {case}", flush=True)
+    comps, kind, m, thr = window_case(case)
+    nt = comps[0].shape[-1]
+    mesh = engine.RegularSimplicialMesh(comps[0].shape)
+    one = engine.extract(mesh, kind, m, comps, thr).to_complex()
+    assert len(one.vertices) > 50
+    for w in (3, 4, nt, nt + 3):
+        print(f"[stream-window] {case} window {w}", flush=True)
+        same(run_stream(comps, kind, m, thr, window=w), one)
+    same(run_stream(comps, kind, m, thr, window=3, retire=0), one)
+
+
+def test_stream_window_is_set_before_the_first_timestep():
+    comps = list(synth.cp2d_config(nx=12, ny=10, nt=3, seed=2))
+    s = engine.Stream(comps[0].shape[:-1], 0, 2, _lib.F64, 2)
+    s.append([np.asfortranarray(c[..., 0]) for c in comps])
+    with pytest.raises(_lib.Ftk2hError) as e:
+        _lib.check(_lib.lib().ftk2h_stream_set_window(s.h, 3))
+    assert e.value.code == 1
+    with pytest.raises(_lib.Ftk2hError):
+        engine.Stream(comps[0].shape[:-1], 0, 2, _lib.F64, 2, window=1)
+    s.close()
 
 
 def test_stream_index_overflow_is_reported(monkeypatch):
