@@ -1092,7 +1092,11 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
           mx = fmax(mx, fabs(X[k][c]));
         }
       }
-      if (inside3_exact_d(X, mx)) atomicOr(&prom[owner], 1ull << t);
+      // A tet whose twelve quantized values are all zero is inside: every level of
+      // the four det3 is zero down to the last, constant one, the same for all four
+      // since the corners are in ascending id order.  Regions that round to zero
+      // hold many such tets; they skip the levels.
+      if (mx == 0.0 || inside3_exact_d(X, mx)) atomicOr(&prom[owner], 1ull << t);
     });
     mk.sol |= prom[threadIdx.x];
   }

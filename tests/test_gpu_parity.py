@@ -97,6 +97,33 @@ def test_cp3d_noise_heavy_float():
     compare("cp3d_float_noise_20x18x16x4", comps, "cp", 3)
 
 
+@pytest.mark.parametrize("sigma,dtype", [((1.5, 2.5), np.float64), ((2.0, 3.0), np.float64),
+                                         ((1.5, 2.5), np.float32)])
+def test_cp3d_zero_regions(sigma, dtype):
+    """Narrow Gaussians: large regions whose values round to zero (12-28% of the
+    vertices), some below the solver's pivot threshold.  Their tets are inside by SoS
+    and the solver gate decides them."""
+    comps = synth.gaussians((24, 22, 20), 4, 4, sigma=sigma, seed=5, dtype=dtype)
+    compare(f"cp3d zero regions {sigma} {np.dtype(dtype).name}", comps, "cp", 3)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cp3d_sub_quantum_linear_field(dtype):
+    """v = 1e-9 A (x - c(t)): every value rounds to zero (|v| Q < 1/2), so every tet
+    is inside by SoS, and the solver gate (pivots ~1e-9 above its 1e-12 threshold)
+    keeps exactly the tets around the moving zero: the nodes all lie in the all-zero
+    region."""
+    A = [[1.0, 0.3, -0.2], [0.1, -1.5, 0.4], [0.2, 0.1, 2.0]]
+    dims, nt, c0, vel = (12, 11, 10), 4, (5.37, 4.21, 4.83), (0.27, -0.19, 0.11)
+    grids = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in dims], np.arange(nt, dtype=np.float64),
+                        indexing="ij")
+    rel = [grids[j] - (c0[j] + vel[j] * grids[3]) for j in range(3)]
+    comps = [np.asfortranarray((1e-9 * sum(A[i][j] * rel[j] for j in range(3))).astype(dtype)) for i in range(3)]
+    assert max(np.abs(c).max() for c in comps) * 1e6 < 0.5
+    cx, ref = compare(f"cp3d sub-quantum linear {np.dtype(dtype).name}", comps, "cp", 3)
+    assert len(ref["verts"]) > 0
+
+
 def test_cp3d_large_magnitudes_exact_path():
     # |v| > 1.05 -> quantized values beyond the fast filter: verbatim exact path
     comps = synth.gaussians((14, 12, 10), 3, 5, sigma=(3, 6), amp=(20.0, 40.0), seed=9)

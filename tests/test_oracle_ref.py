@@ -61,6 +61,27 @@ def test_origin_inside3_random_degenerate(seed):
         assert np.array_equal(a, b), f"dtype {dtype}: {np.flatnonzero(a != b)[:5]}"
 
 
+def test_all_zero_simplices_have_a_constant_answer():
+    """A simplex whose quantized values are all zero, vertices in ascending id order
+    (as every chain simplex of the mesh lists them): origin_inside<3> is always true
+    and origin_inside<2> always false, whatever the ids (every det level is zero and
+    the last, constant one decides).  The 3-D fast path relies on it."""
+    rng = np.random.default_rng(17)
+    n = 20000
+    idx = np.sort(rng.integers(0, 1 << 40, size=(n, 4)).astype(np.uint64), axis=1)
+    keep = np.all(np.diff(idx.astype(np.int64), axis=1) > 0, axis=1)
+    idx = idx[keep]
+    for dtype in (1, 0):
+        a, b = both_inside(np.zeros((len(idx), 4, 3)), idx, dtype)
+        assert a.all() and b.all()
+        # values below half a quantum (|v| Q < 1/2) round to zero too
+        tiny = rng.uniform(-0.49e-6, 0.49e-6, size=(len(idx), 4, 3))
+        a, b = both_inside(tiny, idx, dtype)
+        assert a.all() and b.all()
+        a, b = both_inside(np.zeros((len(idx), 3, 2)), np.ascontiguousarray(idx[:, :3]), dtype)
+        assert not a.any() and not b.any()
+
+
 def test_det3_levels_direct():
     rng = np.random.default_rng(7)
     for _ in range(20000):
