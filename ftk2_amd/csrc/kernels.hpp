@@ -35,6 +35,8 @@ namespace ftk2h {
 
 constexpr int KIND_CP = 0, KIND_CONTOUR = 1, KIND_FIBER = 2;
 constexpr int BLOCK = 256;
+// bsum mark of a tile whose node count stage 1 already knows (counts are below 2^14)
+constexpr uint32_t TILE_DONE = 0x80000000u;
 
 struct KParams {
   int64_t dims[4];     // local extents (1 beyond d)
@@ -1105,6 +1107,16 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
     gptr(P.pend)[v] = mk.sol | mk.deg;
     gptr(P.degm)[v] = mk.deg;
   }
+  // A tile that leaves nothing to stage 2 has its node count now: it goes to bsum
+  // marked TILE_DONE, and stage 2 only clears the mark instead of re-reading the
+  // tile's masks.  Other tiles get 0 until stage 2 writes their count.
+  if (!__syncthreads_or((mk.sol | mk.deg) != 0)) {
+    uint32_t tot;
+    block_excl_scan((uint32_t)__popcll(mk.acc), &tot);
+    if (threadIdx.x == 0) gptr(P.bsum)[blk] = tot | TILE_DONE;
+  } else if (threadIdx.x == 0) {
+    gptr(P.bsum)[blk] = 0;
+  }
 }
 
 // Stage 2: the pending simplices of a tile, spread over the workgroup's lanes by
@@ -1117,6 +1129,14 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
 // keeps a wave on one code path.
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
+  {
+    // a tile finished by stage 1: its count is in bsum already
+    const uint32_t pre = gptr(P.bsum)[blk];
+    if (__syncthreads_or((pre & TILE_DONE) != 0)) {
+      if (threadIdx.x == 0) gptr(P.bsum)[blk] = pre & ~TILE_DONE;
+      return;
+    }
+  }
   __shared__ ItemLds il;
   __shared__ unsigned long long slot[BLOCK];
   __shared__ unsigned long long prom[BLOCK];
@@ -1221,6 +1241,14 @@ __device__ __forceinline__ bool is_hc_base(const KParams& P, const int64_t* l) {
 
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void offsets_tile(const KParams& P, int64_t blk) {
+  if (blk == 0 && threadIdx.x == 0) *P.xcount = 0;
+  // A tile without nodes has no cells either (every cell of hypercube v has a node
+  // owned by v), and no other tile reads its node offsets (only those of vertices
+  // that own an active face), so it only records its cell count, 0.
+  if (gptr(P.bsum)[blk + 1] == gptr(P.bsum)[blk]) {
+    if (threadIdx.x == 0) gptr(P.ebsum)[blk] = 0;
+    return;
+  }
   const int64_t v = blk * BLOCK + threadIdx.x;
   uint64_t m = 0;
   uint32_t nc = 0;
@@ -1235,7 +1263,6 @@ __device__ __forceinline__ void offsets_tile(const KParams& P, int64_t blk) {
   uint32_t tot;
   const uint32_t ex = block_excl_scan((uint32_t)__popcll(m), &tot);
   if (v < P.nvert) gptr(P.node_off)[v] = gptr(P.bsum)[blk] + ex;
-  if (blk == 0 && threadIdx.x == 0) *P.xcount = 0;
   block_excl_scan(nc, &tot);
   if (threadIdx.x == 0) gptr(P.ebsum)[blk] = tot;
 }
@@ -1345,6 +1372,8 @@ template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   constexpr int NTY = Chains<D, M>::N;
   constexpr int NV = CellKind<KIND, D, M>::verts;
+  // a tile without nodes has no cells (k_offsets left its counts unwritten)
+  if (gptr(P.bsum)[blk + 1] == gptr(P.bsum)[blk]) return;
   __shared__ uint16_t nq[EMIT_NQ];  // node queue, then the workgroup union-find
   __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t stage[EMIT_STAGE_WORDS];
