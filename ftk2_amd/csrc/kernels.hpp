@@ -35,8 +35,8 @@ namespace ftk2h {
 
 constexpr int KIND_CP = 0, KIND_CONTOUR = 1, KIND_FIBER = 2;
 constexpr int BLOCK = 256;
-// bsum mark of a tile whose node count stage 1 already knows (counts are below 2^14)
-constexpr uint32_t TILE_DONE = 0x80000000u;
+// bsum of a tile that stage 1 leaves to stage 2 (node counts of a tile are below 2^14)
+constexpr uint32_t TILE_PENDING = 0xFFFFFFFFu;
 
 struct KParams {
   int64_t dims[4];     // local extents (1 beyond d)
@@ -633,6 +633,24 @@ template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void node_values(const KParams& P, const CornerTab& tab, const VtxRef& r, int t,
                                             T (*vals)[M]) {
   const uint32_t pc = tab.pc[t];
+  if (P.packed) {
+    // every component x-fastest with the local extents: one element index per corner
+    int64_t e[M + 1];
+#pragma unroll
+    for (int i = 0; i <= M; ++i) {
+      const int off = i == 0 ? 0 : (int)((pc >> (4 * (i - 1))) & 0xF);
+      e[i] = r.base[0] + tab.coff[0][off];
+    }
+#pragma unroll
+    for (int i = 0; i <= M; ++i)
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        T v = gptr(static_cast<const T*>(P.comp[k]))[e[i]];
+        if constexpr (KIND == KIND_FIBER || KIND == KIND_CONTOUR) v = v - (T)P.thr[k];
+        vals[i][k] = v;
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i <= M; ++i) {
     const int off = i == 0 ? 0 : (int)((pc >> (4 * (i - 1))) & 0xF);
@@ -1107,15 +1125,15 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
     gptr(P.pend)[v] = mk.sol | mk.deg;
     gptr(P.degm)[v] = mk.deg;
   }
-  // A tile that leaves nothing to stage 2 has its node count now: it goes to bsum
-  // marked TILE_DONE, and stage 2 only clears the mark instead of re-reading the
-  // tile's masks.  Other tiles get 0 until stage 2 writes their count.
+  // A tile that leaves nothing to stage 2 has its node count now: it goes to bsum,
+  // and stage 2 skips the tile instead of re-reading its masks.  Other tiles get
+  // TILE_PENDING until stage 2 writes their count.
   if (!__syncthreads_or((mk.sol | mk.deg) != 0)) {
     uint32_t tot;
     block_excl_scan((uint32_t)__popcll(mk.acc), &tot);
-    if (threadIdx.x == 0) gptr(P.bsum)[blk] = tot | TILE_DONE;
+    if (threadIdx.x == 0) gptr(P.bsum)[blk] = tot;
   } else if (threadIdx.x == 0) {
-    gptr(P.bsum)[blk] = 0;
+    gptr(P.bsum)[blk] = TILE_PENDING;
   }
 }
 
@@ -1129,14 +1147,8 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
 // keeps a wave on one code path.
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
-  {
-    // a tile finished by stage 1: its count is in bsum already
-    const uint32_t pre = gptr(P.bsum)[blk];
-    if (__syncthreads_or((pre & TILE_DONE) != 0)) {
-      if (threadIdx.x == 0) gptr(P.bsum)[blk] = pre & ~TILE_DONE;
-      return;
-    }
-  }
+  // a tile finished by stage 1 has its count in bsum already (one uniform load)
+  if (__builtin_amdgcn_readfirstlane(gptr(P.bsum)[blk]) != TILE_PENDING) return;
   __shared__ ItemLds il;
   __shared__ unsigned long long slot[BLOCK];
   __shared__ unsigned long long prom[BLOCK];
@@ -1452,21 +1464,21 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
     const uint64_t cbase = gptr(P.ebsum)[blk];
     if (ctot * NV <= P.stage_cap) {
       __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
-      if (nc) cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
+      // thread whose segment holds each cell (at most EMIT_STAGE_WORDS / 2 cells), in
+      // the LDS of the node phase's vertex references, which are no longer read
+      static_assert(sizeof(refs) >= EMIT_STAGE_WORDS / 2, "cell owners do not fit");
+      uint8_t* cown = reinterpret_cast<uint8_t*>(refs);
+      if (nc) {
+        cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
+        for (uint32_t c = ex; c < ex + nc; ++c) cown[c] = (uint8_t)threadIdx.x;
+      }
       segx[threadIdx.x] = ex;
       if (threadIdx.x == 0) segx[BLOCK] = ctot;
       __syncthreads();
       // each cell goes straight to its sorted place in the workgroup's output range
       auto dst = gptr(P.cells) + cbase * NV;
       for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
-        uint32_t lo = 0, hi = BLOCK;  // segx[lo] <= c < segx[hi]
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) / 2;
-          if (segx[mid] <= c)
-            lo = mid;
-          else
-            hi = mid;
-        }
+        const uint32_t lo = cown[c];
         const uint32_t sb = segx[lo];
         const uint32_t pos = sb + cell_rank<NV>(stage, sb, segx[lo + 1], c);
 #pragma unroll

@@ -6,10 +6,9 @@
 // with the timesteps it has queued): the same mesh with local extent W + 1, so the
 // planes inside the window are classified once instead of twice.  Device-resident
 // state:
-//   * the timesteps of the current window in `wcap` slots per field component; a
-//     window runs over consecutive slots, up or down, and the next one starts at
-//     its last slot going the other way, so a window is a plain strided view (for
-//     wcap = 2, the two halves of a ping-pong buffer);
+//   * the timesteps of the current window in `wcap` consecutive slots per field
+//     component, so a window is one x-fastest array (the kernels' packed layout);
+//     the next window starts with the last timestep of this one, copied to slot 0;
 //   * the previous window's result: its upper-plane nodes (pure spatial simplices
 //     at its last time) are also nodes of the next window, so that window stays
 //     open until the next one arrives and its edges can be re-indexed by key;
@@ -132,8 +131,6 @@ struct ftk2h_stream {
   hipStream_t st = nullptr;
   void* fields[3] = {nullptr, nullptr, nullptr};  // wcap * nsp values each
   int wcap = 2;                                   // planes of a full window (slabs + 1)
-  int64_t s0 = 0;                                 // slot of the window's first plane
-  int dir = 1;                                    // slot step from one plane to the next
   int np = 0;                                     // planes of the window appended so far
   uint64_t t = 0;                                 // timesteps appended
   uint64_t slabs = 0;                             // windows extracted
@@ -290,7 +287,7 @@ namespace ftk2h_internal {
 
 int stream_put(ftk2h_stream* S, const ftk2h_field_view* comps) {
   if (stream_slab_ready(S)) return fail(FTK2H_E_INVALID, "stream: the pending window must be extracted first");
-  const int64_t slot = S->s0 + (int64_t)S->dir * S->np;
+  const int64_t slot = S->np;
   // copy the timestep into its slot (x fastest, contiguous)
   for (int c = 0; c < S->ncomp; ++c) {
     char* dst = static_cast<char*>(S->fields[c]) + slot * S->nsp * S->esz;
@@ -349,8 +346,8 @@ int stream_set_window(ftk2h_stream* S, int planes) {
   return FTK2H_OK;
 }
 
-// The window [t - np, t - 1]: views over its slots (time stride = +-nsp), offset
-// t - np in time, global time extent t.
+// The window [t - np, t - 1]: views over its slots (time stride nsp), offset t - np
+// in time, global time extent t.
 void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_view* views) {
   const uint64_t slab_t = S->t - (uint64_t)S->np;
   std::memset(md, 0, sizeof(*md));
@@ -366,13 +363,13 @@ void stream_slab_desc(const ftk2h_stream* S, ftk2h_mesh_desc* md, ftk2h_field_vi
   for (int c = 0; c < S->ncomp; ++c) {
     views[c] = ftk2h_field_view{};
     views[c].on_device = 1;
-    views[c].ptr = static_cast<char*>(S->fields[c]) + S->s0 * S->nsp * S->esz;
+    views[c].ptr = S->fields[c];
     int64_t s = 1;
     for (int j = 0; j < S->sd; ++j) {
       views[c].strides[j] = s;
       s *= (int64_t)S->spatial[j];
     }
-    views[c].strides[S->sd] = S->dir * (int64_t)S->nsp;
+    views[c].strides[S->sd] = (int64_t)S->nsp;
   }
 }
 
@@ -413,21 +410,14 @@ int stream_take_slab_n(ftk2h_stream* S, ftk2h_result* R, uint32_t nlow) {
   S->open = R;
   S->open_nlow = nlow;
   S->slabs++;
-  // The next window starts at this one's last plane and runs the other way.  After
-  // a partial window fewer than wcap slots lie that way: the plane then moves to
-  // slot 0 first (the extraction that read the slots has completed).
-  const int64_t last = S->s0 + (int64_t)S->dir * (S->np - 1);
-  S->dir = -S->dir;
-  S->s0 = last;
+  // The next window starts with this one's last timestep, moved to slot 0 (the
+  // extraction that read the slots has completed; a copy keeps every window one
+  // ascending array, which the kernels read faster than a negative time stride).
+  const int64_t last = S->np - 1;
   S->np = 1;
-  const int64_t room = S->dir > 0 ? S->wcap - last : last + 1;
-  if (room < S->wcap) {
-    for (int c = 0; c < S->ncomp; ++c) {
-      char* f = static_cast<char*>(S->fields[c]);
-      HIP_TRY(hipMemcpyAsync(f, f + last * S->nsp * S->esz, S->nsp * S->esz, hipMemcpyDeviceToDevice, S->st));
-    }
-    S->s0 = 0;
-    S->dir = 1;
+  for (int c = 0; c < S->ncomp; ++c) {
+    char* f = static_cast<char*>(S->fields[c]);
+    HIP_TRY(hipMemcpyAsync(f, f + last * S->nsp * S->esz, S->nsp * S->esz, hipMemcpyDeviceToDevice, S->st));
   }
   return FTK2H_OK;
 }
