@@ -243,49 +243,95 @@ constexpr const ChainTable<D, M>& CH = Chains<D, M>::table;
 template <int D, int M>
 constexpr const CellFaceTable<D, M>& CF = CellFaces<D, M>::table;
 
-// index of triangle (0,a,b) among Chains<D,2>
-template <int D>
-constexpr int tri_index(int a, int b) {
-  constexpr const auto& C = Chains<D, 2>::table;
-  for (int t = 0; t < C.n; ++t)
-    if (C.s[t][1] == a && C.s[t][2] == b) return t;
-  return -1;
-}
-
 // ---------------------------------------------------------------------------
 // Triple product signs: the error-bounded FP32 filter of filter.hpp (cross3f,
 // dot_signf); a value within the bound sends its simplex to the exact path.
 
-// Tet types ordered by (b, c, a): tets sharing the pair (b, c) are consecutive, so
-// the cross product Xb x Xc is computed once per pair and dies right after.
+// Bit-sliced tet logic of classify_cp3.  Triangle (0,a,b) and tet (a,b,c) values
+// are computed in type order (types are sorted by their chain, so the tets sharing
+// (a,b) are consecutive and follow triangle (a,b)); each value's bit is shifted into
+// 32-bit words as it is computed (w = (w << 1) | bit), so item i of n lies in word
+// i >> 5 at bit push_bit(i, n).  A tet needs the bits of three triangles, (0,a,b),
+// (0,a,c) and (0,b,c) ("slots" 0, 1, 2), at its own bit position.  Every such move
+// shifts the 64 triangle bits by d = (tet bit) - (triangle bit); the moves of one
+// slot and tet word with the same d form one group, gathered by one shift and one
+// masked OR.  The uncertain bits of all three slots are ORed together, so their
+// groups are merged across slots.
+constexpr int push_bit(int i, int n) {
+  const int w = i >> 5;
+  const int in_word = n - 32 * w < 32 ? n - 32 * w : 32;
+  return in_word - 1 - (i & 31);
+}
+
 template <int D>
-struct TetOrder {
-  uint8_t t[64];
+struct SliceTab {
+  int nsg[3][2] = {};           // sign groups per slot and tet word
+  int sd[3][2][32] = {};        // group shift d: tet bit q takes triangle bit q - d
+  uint32_t sm[3][2][32] = {};   // tet bits of the group
+  int nug[2] = {};              // uncertainty groups per tet word (slots merged)
+  int ud[2][32] = {};
+  uint32_t um[2][32] = {};
+  uint32_t axis[4][2] = {};     // tets whose last corner raises axis j
+  uint8_t first[64] = {};       // first tet (a, b, c) of triangle (a, b), and their count
+  uint8_t count[64] = {};
 };
+
 template <int D>
-constexpr TetOrder<D> build_tet_order() {
-  TetOrder<D> o{};
-  const auto C = build_chains<D, 3>();
-  bool used[64] = {};
-  for (int k = 0; k < C.n; ++k) {
-    int best = -1;
-    for (int t = 0; t < C.n; ++t) {
-      if (used[t]) continue;
-      if (best < 0) {
-        best = t;
-        continue;
-      }
-      const int kb = (C.s[best][2] << 8) | (C.s[best][3] << 4) | C.s[best][1];
-      const int kt = (C.s[t][2] << 8) | (C.s[t][3] << 4) | C.s[t][1];
-      if (kt < kb) best = t;
+constexpr SliceTab<D> build_slice_tab() {
+  SliceTab<D> S{};
+  const auto C2 = build_chains<D, 2>();
+  const auto C3 = build_chains<D, 3>();
+  auto tri = [&](int a, int b) {
+    for (int t = 0; t < C2.n; ++t)
+      if (C2.s[t][1] == a && C2.s[t][2] == b) return t;
+    return -1;
+  };
+  for (int k = 0; k < C3.n; ++k) {
+    const int a = C3.s[k][1], b = C3.s[k][2], c = C3.s[k][3];
+    const int w = k >> 5, q = push_bit(k, C3.n);
+    const int t3[3] = {tri(a, b), tri(a, c), tri(b, c)};
+    for (int s = 0; s < 3; ++s) {
+      const int p = 32 * (t3[s] >> 5) + push_bit(t3[s], C2.n);
+      const int d = q - p;
+      int g = 0;
+      while (g < S.nsg[s][w] && S.sd[s][w][g] != d) ++g;
+      if (g == S.nsg[s][w]) S.sd[s][w][S.nsg[s][w]++] = d;
+      S.sm[s][w][g] |= 1u << q;
+      int u = 0;
+      while (u < S.nug[w] && S.ud[w][u] != d) ++u;
+      if (u == S.nug[w]) S.ud[w][S.nug[w]++] = d;
+      S.um[w][u] |= 1u << q;
     }
-    used[best] = true;
-    o.t[k] = (uint8_t)best;
+    for (int j = 0; j < D; ++j)
+      if ((c >> j) & 1) S.axis[j][w] |= 1u << q;
+    const int ab = t3[0];
+    if (S.count[ab] == 0) S.first[ab] = (uint8_t)k;
+    S.count[ab]++;
   }
-  return o;
+  return S;
 }
 template <int D>
-constexpr TetOrder<D> TET_ORDER = build_tet_order<D>();
+constexpr SliceTab<D> SLICE = build_slice_tab<D>();
+
+// Bits [q - d] of the 64-bit word, aligned to bit q (a compile-time funnel shift).
+template <int d>
+__device__ __forceinline__ uint32_t shifted64(uint64_t w) {
+  if constexpr (d >= 0)
+    return (uint32_t)(w << d);
+  else
+    return (uint32_t)(w >> -d);
+}
+
+// Items pushed one bit at a time (push_bit layout) back to bit i = item i.
+template <int N>
+__device__ __forceinline__ uint64_t unpush(const uint32_t* w) {
+  if constexpr (N <= 32) {
+    return (uint64_t)(__builtin_bitreverse32(w[0]) >> (32 - N));
+  } else {
+    return (uint64_t)__builtin_bitreverse32(w[0]) |
+           ((uint64_t)(__builtin_bitreverse32(w[1]) >> (64 - N)) << 32);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Per-vertex classification, two stages.
@@ -318,23 +364,60 @@ struct Tile3 {
   static constexpr int FLOATS = NSEG * SEG * 3;
 };
 
-template <class T, int D>
-__device__ __forceinline__ void stage_quantized3(const KParams& P, int64_t v0, float* tile) {
-  for (int e = threadIdx.x; e < Tile3<D>::NSEG * Tile3<D>::SEG; e += BLOCK) {
-    const int seg = e / Tile3<D>::SEG, i = e - seg * Tile3<D>::SEG;
-    int64_t u = v0 + P.loff[seg << 1] + i;
-    u = u < P.nvert ? u : P.nvert - 1;  // such corners lie outside the grid and are masked
-    VtxRef r;
-    if (P.packed) {
-      r.base[0] = r.base[1] = r.base[2] = u;
-    } else {
+// Loads of one LDS tile: element e is vertex e % SEG of run e / SEG.  Every load of
+// the thread's elements is issued before any value is used (a loop that stores each
+// element before loading the next waits for memory once per element).  Element e of
+// unrolled step `it` (e = thread + it * BLOCK) lies in run it or it - 1, so the run
+// offset is one of two compile-time kernel-argument entries.
+template <class T, int D, int NCOMP, class F>
+__device__ __forceinline__ void stage_tile(const KParams& P, int64_t v0, F&& store) {
+  constexpr int NSEG = Tile3<D>::NSEG, SEG = Tile3<D>::SEG, NE = NSEG * SEG;
+  constexpr int IT = (NE + BLOCK - 1) / BLOCK;
+  if (!P.packed) {
+    // strided views: one element at a time (the index decomposition dominates)
+    for (int e = threadIdx.x; e < NE; e += BLOCK) {
+      const int seg = e / SEG;
+      int64_t u = v0 + P.loff[seg << 1] + (e - seg * SEG);
+      u = u < P.nvert ? u : P.nvert - 1;  // such corners lie outside the grid and are masked
       int64_t l[4];
       decompose<D>(P, u, l);
-      r = make_ref<D>(P, u, l);
-    }
+      const VtxRef r = make_ref<D>(P, u, l);
+      T v[NCOMP];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) tile[e * 3 + k] = (float)qt(load_val<T>(P, k, r, 0), P.Q);
+      for (int k = 0; k < NCOMP; ++k) v[k] = load_val<T>(P, k, r, 0);
+      store(e, v);
+    }
+    return;
   }
+  T val[IT][NCOMP];
+  sfor<0, IT>([&](auto ii) {
+    constexpr int it = decltype(ii)::value;
+    constexpr int s_hi = it < NSEG ? it : NSEG - 1, s_lo = it > 0 ? it - 1 : 0;
+    const int e = (int)threadIdx.x + it * BLOCK;
+    if (e < NE) {
+      const bool lo = (int)threadIdx.x < it;  // e < it * SEG
+      const int seg = lo ? s_lo : s_hi;
+      int64_t u = v0 + (lo ? P.loff[s_lo << 1] : P.loff[s_hi << 1]) + (e - seg * SEG);
+      u = u < P.nvert ? u : P.nvert - 1;  // such corners lie outside the grid and are masked
+      VtxRef r;
+      r.base[0] = r.base[1] = r.base[2] = u;
+#pragma unroll
+      for (int k = 0; k < NCOMP; ++k) val[it][k] = load_val<T>(P, k, r, 0);
+    }
+  });
+  sfor<0, IT>([&](auto ii) {
+    constexpr int it = decltype(ii)::value;
+    const int e = (int)threadIdx.x + it * BLOCK;
+    if (e < NE) store(e, val[it]);
+  });
+}
+
+template <class T, int D>
+__device__ __forceinline__ void stage_quantized3(const KParams& P, int64_t v0, float* tile) {
+  stage_tile<T, D, 3>(P, v0, [&](int e, const T* v) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tile[e * 3 + k] = (float)qt(v[k], P.Q);
+  });
 }
 
 // The same tile for the det2 predicates (CriticalPointPredicate<2,T>, and
@@ -343,117 +426,127 @@ __device__ __forceinline__ void stage_quantized3(const KParams& P, int64_t v0, f
 // workgroup instead of once per vertex whose hypercube uses it.
 template <int KIND, class T, int D>
 __device__ __forceinline__ void stage_quantized2(const KParams& P, int64_t v0, double* tile) {
-  for (int e = threadIdx.x; e < Tile3<D>::NSEG * Tile3<D>::SEG; e += BLOCK) {
-    const int seg = e / Tile3<D>::SEG, i = e - seg * Tile3<D>::SEG;
-    int64_t u = v0 + P.loff[seg << 1] + i;
-    u = u < P.nvert ? u : P.nvert - 1;  // such corners lie outside the grid and are masked
-    VtxRef r;
-    if (P.packed) {
-      r.base[0] = r.base[1] = r.base[2] = u;
-    } else {
-      int64_t l[4];
-      decompose<D>(P, u, l);
-      r = make_ref<D>(P, u, l);
-    }
+  stage_tile<T, D, 2>(P, v0, [&](int e, const T* v) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      T v = load_val<T>(P, k, r, 0);
-      if constexpr (KIND == KIND_FIBER) v = v - (T)P.thr[k];
-      tile[e * 2 + k] = qt(v, P.Q);
+      T w = v[k];
+      if constexpr (KIND == KIND_FIBER) w = w - (T)P.thr[k];
+      tile[e * 2 + k] = qt(w, P.Q);
     }
-  }
-}
-
-// Bit i of a sequence of N flags shifted into 32-bit words one at a time
-// (w = (w << 1) | flag, flags 32j.. in word j): the last flag of a word is bit 0.
-template <int N, int I>
-__device__ __forceinline__ uint32_t pushed_bit(const uint32_t* w) {
-  constexpr int word = I >> 5;
-  constexpr int in_word = (N - 32 * word) < 32 ? (N - 32 * word) : 32;
-  constexpr int pos = in_word - 1 - (I & 31);
-  return (w[word] >> pos) & 1u;
+  });
 }
 
 // Critical points of a 3-component field on 3-simplices (CriticalPointPredicate<3,T>,
 // predicate.hpp:66-99).
 template <class T, int D>
 __device__ __forceinline__ Masks classify_cp3(const KParams& P, const Vtx& x, const float* tile) {
-  uint64_t acc = 0, sol = 0, deg = 0;
-  constexpr int NC = 1 << D;
+  constexpr int NC = 1 << D, FULL = NC - 1;
   const uint32_t inb = corner_mask<D>(P, x.l);
+  // Corners outside the grid read some other vertex's value from the tile; a tet
+  // whose last corner c is inside has all its corners inside (they are subsets of
+  // c), so only the tet masks need the in-grid gate, and their values only raise mx
+  // (a larger filter bound is still a bound).
   float X[NC][3];
   float mx = 0.f;  // the largest |q|, exact (integers below 2^24 are exact in float)
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
-    const bool in = (inb >> s) & 1;
     const float* src = tile + ((s >> 1) * Tile3<D>::SEG + threadIdx.x + (s & 1)) * 3;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      X[s][k] = in ? src[k] : 0.f;
+      X[s][k] = src[k];
       mx = fmaxf(mx, fabsf(X[s][k]));
     }
   }
   if (!(mx < 0x1p20f)) {
     // magnitudes beyond the fast filter: every tet takes the verbatim path
+    uint64_t deg = 0;
     sfor<0, CH<D, 3>.n>([&](auto tt) {
       constexpr int t = decltype(tt)::value;
       constexpr int c = CH<D, 3>.s[t][3];
       if ((inb >> c) & 1) deg |= 1ull << t;
     });
-    return Masks{acc, sol, deg};
+    return Masks{0, 0, deg};
   }
   const float bnd = mx * mx * mx * FTK2H_TP32_BOUND;  // mx^3 < 2^60; rounding well inside the 2x margin
   // Each triple product f gives two bits, shifted into 32-bit words in the order
   // computed (one v_alignbit each): its sign bit (the exact sign when f is certain)
   // and "certain" = the sign bit of bnd - |f|, set iff |f| > bnd (a difference of
   // two floats is zero only when they are equal and keeps its sign when flushed).
-  // Corners outside the grid hold 0; a tet whose last corner c is inside has all its
-  // corners inside (they are subsets of c), so only the tet masks need the inb gate.
+  // tp(0,a,b) = (Xa x Xb) . X0 and tp(a,b,c) = (Xa x Xb) . Xc share the cross
+  // product of (a, b); for b = FULL (no tet has it as its middle corner)
+  // tp(0,a,FULL) = (X_FULL x X0) . Xa.
   constexpr int NTRI = CH<D, 2>.n, NTET = CH<D, 3>.n;
-  uint32_t tn[2] = {0, 0}, tc[2] = {0, 0};  // triangles (0, a, b), by chain type
+  uint32_t tn[2] = {0, 0}, tc[2] = {0, 0};  // triangles (0, a, b), by type
+  uint32_t fn[2] = {0, 0}, fc[2] = {0, 0};  // tets' opposite faces (a, b, c), by type
+  float ef[3];
+  cross3f(X[FULL], X[0], ef);
   sfor<0, NTRI>([&](auto tt) {
     constexpr int t = decltype(tt)::value;
     constexpr int a = CH<D, 2>.s[t][1], b = CH<D, 2>.s[t][2];
     float e[3];
-    cross3f(X[0], X[a], e);
-    const float f = dot3f(e, X[b]);
+    float f;
+    if constexpr (b == FULL) {
+      f = dot3f(ef, X[a]);
+    } else {
+      cross3f(X[a], X[b], e);
+      f = dot3f(e, X[0]);
+    }
     tn[t >> 5] = __builtin_amdgcn_alignbit(tn[t >> 5], __float_as_uint(f), 31);
     tc[t >> 5] = __builtin_amdgcn_alignbit(tc[t >> 5], __float_as_uint(bnd - fabsf(f)), 31);
+    if constexpr (b != FULL) {
+      sfor<SLICE<D>.first[t], SLICE<D>.first[t] + SLICE<D>.count[t]>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        constexpr int c = CH<D, 3>.s[k][3];
+        const float g = dot3f(e, X[c]);
+        fn[k >> 5] = __builtin_amdgcn_alignbit(fn[k >> 5], __float_as_uint(g), 31);
+        fc[k >> 5] = __builtin_amdgcn_alignbit(fc[k >> 5], __float_as_uint(bnd - fabsf(g)), 31);
+      });
+    }
     __builtin_amdgcn_sched_barrier(0);  // keep live ranges short
   });
-  // opposite faces tp(a,b,c), in (b,c)-grouped order k
-  uint32_t fn[2] = {0, 0}, fc[2] = {0, 0};
-  sfor<0, NTET>([&](auto kk) {
-    constexpr int k = decltype(kk)::value;
-    constexpr int t = TET_ORDER<D>.t[k];
-    constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
-    float e[3];
-    cross3f(X[b], X[c], e);
-    const float f = dot3f(e, X[a]);  // tp(a,b,c) = (Xb x Xc) . Xa
-    fn[k >> 5] = __builtin_amdgcn_alignbit(fn[k >> 5], __float_as_uint(f), 31);
-    fc[k >> 5] = __builtin_amdgcn_alignbit(fc[k >> 5], __float_as_uint(bnd - fabsf(f)), 31);
-    __builtin_amdgcn_sched_barrier(0);
+  // Gather each tet's three triangle bits to its own position (see SliceTab).
+  const uint64_t sn = ((uint64_t)tn[1] << 32) | tn[0];
+  const uint64_t su = ~(((uint64_t)tc[1] << 32) | tc[0]);  // uncertain triangles
+  constexpr int NW = (NTET + 31) / 32;
+  uint32_t nw[3][2] = {}, uw[2] = {};
+  sfor<0, NW>([&](auto ww) {
+    constexpr int w = decltype(ww)::value;
+    sfor<0, 3>([&](auto ss) {
+      constexpr int s = decltype(ss)::value;
+      sfor<0, SLICE<D>.nsg[s][w]>([&](auto gg) {
+        constexpr int g = decltype(gg)::value;
+        constexpr uint32_t m = SLICE<D>.sm[s][w][g];
+        nw[s][w] |= shifted64<SLICE<D>.sd[s][w][g]>(sn) & m;
+      });
+    });
+    sfor<0, SLICE<D>.nug[w]>([&](auto gg) {
+      constexpr int g = decltype(gg)::value;
+      constexpr uint32_t m = SLICE<D>.um[w][g];
+      uw[w] |= shifted64<SLICE<D>.ud[w][g]>(su) & m;
+    });
   });
   // d3 = det3(v0,v1,v2) = tp(0ab); d2 = det3(v0,v3,v1) = -tp(0ac);
   // d1 = det3(v0,v2,v3) = tp(0bc); d0 = det3(v1,v3,v2) = -tp(abc).
   // Inside by level 0 iff all four are equal; any uncertain/zero value -> verbatim.
-  // With all four certain, each comparison below is an XOR of two sign bits.
-  sfor<0, NTET>([&](auto kk) {
-    constexpr int k = decltype(kk)::value;
-    constexpr int t = TET_ORDER<D>.t[k];
-    constexpr int a = CH<D, 3>.s[t][1], b = CH<D, 3>.s[t][2], c = CH<D, 3>.s[t][3];
-    constexpr int i012 = tri_index<D>(a, b), i013 = tri_index<D>(a, c), i023 = tri_index<D>(b, c);
-    const uint64_t valid = (inb >> c) & 1;
-    const uint32_t cert = pushed_bit<NTRI, i012>(tc) & pushed_bit<NTRI, i013>(tc) & pushed_bit<NTRI, i023>(tc) &
-                          pushed_bit<NTET, k>(fc);
-    const uint32_t n012 = pushed_bit<NTRI, i012>(tn), n013 = pushed_bit<NTRI, i013>(tn);
-    const uint32_t n023 = pushed_bit<NTRI, i023>(tn), nabc = pushed_bit<NTET, k>(fn);
-    // d3 == d2 <=> n012 != n013;  d1 == d3 <=> n023 == n012;  d0 == d3 <=> nabc != n012
-    const uint32_t in = (n012 ^ n013) & ~(n023 ^ n012) & (nabc ^ n012) & 1;
-    deg |= (valid & (uint64_t)(~cert & 1)) << t;
-    sol |= (valid & (uint64_t)(cert & in)) << t;
+  // With all four certain, each comparison is an XOR of two sign bits:
+  // d3 == d2 <=> n012 != n013;  d1 == d3 <=> n023 == n012;  d0 == d3 <=> nabc != n012.
+  uint32_t solw[2] = {0, 0}, degw[2] = {0, 0};
+  sfor<0, NW>([&](auto ww) {
+    constexpr int w = decltype(ww)::value;
+    uint32_t invalid = 0;
+    sfor<0, D>([&](auto jj) {
+      constexpr int j = decltype(jj)::value;
+      constexpr uint32_t ax = SLICE<D>.axis[j][w];
+      invalid |= ((inb >> (1 << j)) & 1) ? 0u : ax;
+    });
+    const uint32_t n012 = nw[0][w], n013 = nw[1][w], n023 = nw[2][w], nabc = fn[w];
+    const uint32_t in = (n012 ^ n013) & ~(n023 ^ n012) & (nabc ^ n012);
+    const uint32_t cert = ~uw[w] & fc[w];
+    solw[w] = ~invalid & cert & in;
+    degw[w] = ~invalid & ~cert;
   });
-  return Masks{acc, sol, 0, deg};  // the verbatim levels run from the tile
+  // the verbatim levels run from the tile
+  return Masks{0, unpush<NTET>(solw), 0, unpush<NTET>(degw)};
 }
 
 // det2-based predicates on 2-simplices: CriticalPointPredicate<2,T> and
