@@ -166,6 +166,47 @@ __device__ __forceinline__ int det3_levels_d(const double* X0, const double* X1,
   if ((d = a0) != 0) return sgnd(d);                          // t=13
   return 1;                                                   // t=14
 }
+// origin_inside<3> for a tet with exactly one zero vertex, on integer-valued doubles
+// |q| < 2^20 (mx = max |q|): 1 inside, 0 outside, -1 when this shortcut does not
+// decide it (the caller then runs inside3_exact_d).  In det3_levels_d a zero row
+// makes level 0 and every level that uses the row zero, so the first level left is
+// a 2x2 minor of the other two rows: level 1 (b0 c1 - b1 c0) for a zero first row,
+// level 4 (-(a0 c1 - a1 c0)) for a zero second row, level 10 (a0 b1 - a1 b0) for a
+// zero third row; such minors (products below 2^40) are exact in FP64, and the
+// shortcut applies when all three are nonzero.  The fourth det3, the one without
+// the zero row, must be decided at level 0 by its error-bounded FP64 value.  With
+// m_pq = Xp.x Xq.y - Xp.y Xq.x the four dets of inside3_exact_d are, by zero vertex z:
+//   d0 = det3(X1,X3,X2): z=1 -m23, z=3 -m12, z=2 +m13, z=0 -det3(X1,X2,X3)
+//   d1 = det3(X0,X2,X3): z=0 +m23, z=2 -m03, z=3 +m02, z=1 +det3(X0,X2,X3)
+//   d2 = det3(X0,X3,X1): z=0 -m13, z=3 -m01, z=1 +m03, z=2 -det3(X0,X1,X3)
+//   d3 = det3(X0,X1,X2): z=0 +m12, z=1 -m02, z=2 +m01, z=3 +det3(X0,X1,X2)
+__device__ __forceinline__ int inside3_one_zero(const double (*X)[3], double mx) {
+  bool z[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) z[i] = X[i][0] == 0.0 && X[i][1] == 0.0 && X[i][2] == 0.0;
+  if ((int)z[0] + (int)z[1] + (int)z[2] + (int)z[3] != 1) return -1;
+  const int zi = z[0] ? 0 : (z[1] ? 1 : (z[2] ? 2 : 3));
+  auto m = [&](int p, int q) { return fma(X[p][0], X[q][1], -(X[p][1] * X[q][0])); };
+  const double m01 = m(0, 1), m02 = m(0, 2), m03 = m(0, 3), m12 = m(1, 2), m13 = m(1, 3), m23 = m(2, 3);
+  // det3 of the three nonzero rows in ascending order, at level 0
+  const double* A = zi == 0 ? X[1] : X[0];
+  const double* B = zi <= 1 ? X[2] : X[1];
+  const double* C = zi <= 2 ? X[3] : X[2];
+  const double e0 = fma(B[1], C[2], -(B[2] * C[1]));  // exact: products below 2^40
+  const double e1 = fma(B[2], C[0], -(B[0] * C[2]));
+  const double e2 = fma(B[0], C[1], -(B[1] * C[0]));
+  double f = e0 * A[0];
+  f = fma(e1, A[1], f);
+  f = fma(e2, A[2], f);
+  if (!(fabs(f) > mx * mx * mx * 0x1p-48)) return -1;
+  const bool pos = (zi == 0 || zi == 2) ? f < 0.0 : f > 0.0;
+  const double a = zi == 0 ? m23 : (zi == 1 ? -m23 : (zi == 2 ? m13 : -m12));
+  const double b = zi == 0 ? -m13 : (zi == 1 ? m03 : (zi == 2 ? -m03 : m02));
+  const double c = zi == 0 ? m12 : (zi == 1 ? -m02 : (zi == 2 ? m01 : -m01));
+  if (a == 0.0 || b == 0.0 || c == 0.0) return -1;
+  return ((a > 0.0) == pos && (b > 0.0) == pos && (c > 0.0) == pos) ? 1 : 0;
+}
+
 // origin_inside<3> on integer-valued doubles, all |q| < 2^20 (mx = max |q|).
 __device__ __forceinline__ bool inside3_exact_d(const double (*X)[3], double mx) {
   const double bnd = mx * mx * mx * 0x1p-48;

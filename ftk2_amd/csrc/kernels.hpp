@@ -1188,12 +1188,13 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
     // sign levels on the exact quantized values of the tile (no gathers)
     __shared__ ItemLds il;
     __shared__ unsigned long long prom[BLOCK];
+    __shared__ unsigned long long hard[BLOCK];
     __shared__ uint32_t pct[64];
     if (threadIdx.x < 64) pct[threadIdx.x] = packed_chain<D, 3>(threadIdx.x);
     prom[threadIdx.x] = 0;
-    run_items(il, mk.tdeg, [&](int owner, int t) {
+    hard[threadIdx.x] = 0;
+    auto corners = [&](int owner, int t, double (*X)[3]) {
       const uint32_t pc = pct[t];
-      double X[4][3];
       double mx = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -1205,12 +1206,30 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
           mx = fmax(mx, fabs(X[k][c]));
         }
       }
-      // A tet whose twelve quantized values are all zero is inside: every level of
-      // the four det3 is zero down to the last, constant one, the same for all four
-      // since the corners are in ascending id order.  Regions that round to zero
-      // hold many such tets; they skip the levels.
-      if (mx == 0.0 || inside3_exact_d(X, mx)) atomicOr(&prom[owner], 1ull << t);
+      return mx;
+    };
+    // First pass, branch-free: a tet whose twelve quantized values are all zero is
+    // inside (every level of the four det3 is zero down to the last, constant one,
+    // the same for all four since the corners are in ascending id order; regions
+    // that round to zero hold many such tets), and most of the others have one zero
+    // vertex (inside3_one_zero).  The rest take the full level cascade in a second
+    // pass, spread over the lanes again, so that its branches cost only its items.
+    run_items(il, mk.tdeg, [&](int owner, int t) {
+      double X[4][3];
+      const double mx = corners(owner, t, X);
+      const int r = mx == 0.0 ? 1 : inside3_one_zero(X, mx);
+      if (r > 0)
+        atomicOr(&prom[owner], 1ull << t);
+      else if (r < 0)
+        atomicOr(&hard[owner], 1ull << t);
     });
+    const uint64_t hw = hard[threadIdx.x];
+    if (__syncthreads_or(hw != 0))
+      run_items(il, hw, [&](int owner, int t) {
+        double X[4][3];
+        const double mx = corners(owner, t, X);
+        if (inside3_exact_d(X, mx)) atomicOr(&prom[owner], 1ull << t);
+      });
     mk.sol |= prom[threadIdx.x];
   }
   if (v < P.nvert) {
