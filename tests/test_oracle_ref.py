@@ -182,3 +182,40 @@ def test_cofaces_pass_adds_no_union(seed):
                 assert track[j] == track[i], (v, nb)
                 checked += 1
     assert checked > len(verts)
+
+
+def test_golden52_inside_set_from_the_reference_sos():
+    """The CP3D float case of test_examples.cpp:153-186 (24^3 x 8, golden 52).  Every
+    tetrahedron of the mesh (the reference's own LUT chains from every vertex, all four
+    corners in the grid: every face of every Kuhn cell) goes through
+    the reference's compiled origin_inside<3> and the oracle's: both accept the same
+    336, and the oracle's solver gate keeps 29 of them, the node count the oracle's
+    engine gives for this case (test_oracle_engine.py::test_cp3d_linear_float).  Only
+    29 of the 336 have corner values of both signs (or zero) in every component: with
+    a zero component the det3 levels decide by minors that need not bound the origin,
+    so no geometric prefilter is used."""
+    import refcases
+    comps = refcases.cp3d_linear_float()
+    dims = np.array(comps[0].shape, dtype=np.int64)
+    vals = np.stack([c.ravel(order="F").astype(np.float64) for c in comps], axis=1)  # x fastest
+    base = np.stack(np.meshgrid(*[np.arange(n) for n in dims], indexing="ij"), axis=-1).reshape(-1, 4, order="F")
+    stride = np.array([1, dims[0], dims[0] * dims[1], dims[0] * dims[1] * dims[2]], dtype=np.int64)
+    L = ref.lut(4, 3)
+    assert L.shape == (60, 4, 4)
+    n_tets, inside_v = 0, []
+    for t in range(len(L)):
+        corners = base[:, None, :] + L[t][None, :, :]  # (n, 4 vertices, 4 axes)
+        ok = np.all(corners < dims, axis=(1, 2))
+        ids = (corners[ok] * stride).sum(axis=2)
+        assert np.all(np.diff(ids, axis=1) > 0)  # ascending vertex ids
+        v = vals[ids]  # (n, 4, 3)
+        a = ref.origin_inside_many(v, ids.astype(np.uint64), dtype=0)
+        b = oracle.origin_inside_many(v, ids.astype(np.uint64), dtype=0)
+        assert np.array_equal(a, b), t
+        n_tets += len(ids)
+        inside_v.append(v[a])
+    inside_v = np.concatenate(inside_v)
+    # a type whose top corner raises axis j has dims_j - 1 positions along j, else dims_j
+    assert n_tets == sum(int(np.prod(dims - L[t][3])) for t in range(len(L)))
+    assert len(inside_v) == 336
+    assert sum(oracle.solve(x, dtype=0)[0] for x in inside_v) == 29
