@@ -1576,13 +1576,18 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
     const uint64_t cbase = gptr(P.ebsum)[blk];
     if (ctot * NV <= P.stage_cap) {
       __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
-      // thread whose segment holds each cell (at most EMIT_STAGE_WORDS / 2 cells), in
-      // the LDS of the node phase's vertex references, which are no longer read
+      // Star edges (two or three per vertex): the thread whose segment holds each
+      // cell (at most EMIT_STAGE_WORDS / 2 cells) in the LDS of the node phase's
+      // vertex references, which are no longer read.  Marching cells come many per
+      // hypercube, where one thread's serial byte stores cost more than a binary
+      // search per cell (contour emit 45.5 -> 66.7 ms with the stores).
+      constexpr bool OWNERS = NV == 2;
       static_assert(sizeof(refs) >= EMIT_STAGE_WORDS / 2, "cell owners do not fit");
       uint8_t* cown = reinterpret_cast<uint8_t*>(refs);
       if (nc) {
         cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
-        for (uint32_t c = ex; c < ex + nc; ++c) cown[c] = (uint8_t)threadIdx.x;
+        if constexpr (OWNERS)
+          for (uint32_t c = ex; c < ex + nc; ++c) cown[c] = (uint8_t)threadIdx.x;
       }
       segx[threadIdx.x] = ex;
       if (threadIdx.x == 0) segx[BLOCK] = ctot;
@@ -1590,7 +1595,20 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       // each cell goes straight to its sorted place in the workgroup's output range
       auto dst = gptr(P.cells) + cbase * NV;
       for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
-        const uint32_t lo = cown[c];
+        uint32_t lo;
+        if constexpr (OWNERS) {
+          lo = cown[c];
+        } else {
+          uint32_t hi = BLOCK;  // segx[lo] <= c < segx[hi]
+          lo = 0;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (segx[mid] <= c)
+              lo = mid;
+            else
+              hi = mid;
+          }
+        }
         const uint32_t sb = segx[lo];
         const uint32_t pos = sb + cell_rank<NV>(stage, sb, segx[lo + 1], c);
 #pragma unroll
