@@ -35,8 +35,11 @@ namespace ftk2h {
 
 constexpr int KIND_CP = 0, KIND_CONTOUR = 1, KIND_FIBER = 2;
 constexpr int BLOCK = 256;
-// bsum of a tile that stage 1 leaves to stage 2 (node counts of a tile are below 2^14)
+// bsum of a tile that stage 1 leaves to stage 2 (node counts of a tile are below
+// 2^14): TILE_PENDING_SOL when its only pending work is the solver gate of SoS-inside
+// simplices (stage 1 then writes only pend), TILE_PENDING otherwise.
 constexpr uint32_t TILE_PENDING = 0xFFFFFFFFu;
+constexpr uint32_t TILE_PENDING_SOL = 0xFFFFFFFEu;
 
 struct KParams {
   int64_t dims[4];     // local extents (1 beyond d)
@@ -1268,20 +1271,30 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
       });
     mk.sol |= prom[threadIdx.x];
   }
-  if (v < P.nvert) {
-    gptr(P.mask)[v] = mk.acc;
-    gptr(P.pend)[v] = mk.sol | mk.deg;
-    gptr(P.degm)[v] = mk.deg;
-  }
   // A tile that leaves nothing to stage 2 has its node count now: it goes to bsum,
-  // and stage 2 skips the tile instead of re-reading its masks.  Other tiles get
-  // TILE_PENDING until stage 2 writes their count.
-  if (!__syncthreads_or((mk.sol | mk.deg) != 0)) {
+  // and stage 2 skips the tile.  Other tiles get a TILE_PENDING sentinel until
+  // stage 2 writes their count; when a pending tile has no accepted and no verbatim
+  // bits (the usual critical-point tile), mask and degm would be zero and only pend
+  // is written (stage 2 writes the final mask).
+  const bool pending = __syncthreads_or((mk.sol | mk.deg) != 0);
+  const bool sol_only = pending && !__syncthreads_or((mk.acc | mk.deg) != 0);
+  if (v < P.nvert) {
+    if (sol_only) {
+      gptr(P.pend)[v] = mk.sol;
+    } else {
+      gptr(P.mask)[v] = mk.acc;
+      if (pending) {
+        gptr(P.pend)[v] = mk.sol | mk.deg;
+        gptr(P.degm)[v] = mk.deg;
+      }
+    }
+  }
+  if (!pending) {
     uint32_t tot;
     block_excl_scan((uint32_t)__popcll(mk.acc), &tot);
     if (threadIdx.x == 0) gptr(P.bsum)[blk] = tot;
   } else if (threadIdx.x == 0) {
-    gptr(P.bsum)[blk] = TILE_PENDING;
+    gptr(P.bsum)[blk] = sol_only ? TILE_PENDING_SOL : TILE_PENDING;
   }
 }
 
@@ -1296,7 +1309,9 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
   // a tile finished by stage 1 has its count in bsum already (one uniform load)
-  if (__builtin_amdgcn_readfirstlane(gptr(P.bsum)[blk]) != TILE_PENDING) return;
+  const uint32_t b0 = __builtin_amdgcn_readfirstlane(gptr(P.bsum)[blk]);
+  if (b0 < TILE_PENDING_SOL) return;
+  const bool full = b0 == TILE_PENDING;  // else only pend was written
   __shared__ ItemLds<2048> il;
   __shared__ unsigned long long slot[BLOCK];
   __shared__ unsigned long long prom[BLOCK];
@@ -1306,9 +1321,11 @@ __device__ __forceinline__ void stage2_tile(const KParams& P, int64_t blk) {
   const int64_t v = blk * BLOCK + threadIdx.x;
   uint64_t acc = 0, w = 0, deg = 0;
   if (v < P.nvert) {
-    acc = gptr(P.mask)[v];
     w = gptr(P.pend)[v];
-    deg = gptr(P.degm)[v];
+    if (full) {
+      acc = gptr(P.mask)[v];
+      deg = gptr(P.degm)[v];
+    }
   }
   if (__syncthreads_or(w != 0)) {
     if (w) refs[threadIdx.x] = make_vtx<D>(P, v).r;
