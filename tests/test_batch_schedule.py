@@ -3,8 +3,10 @@ for every schedule the autotuner tries (ftk2h_batch_autotune's 13 (grid per CU, 
 candidates) and tile counts around chunk and grid multiples, every tile of
 [0, ntiles) goes to exactly one workgroup, none beyond, however the workgroups'
 atomic grabs interleave; the counters start at zero on every run (kb_zero).  This
-rules the schedule out as the cause of the fault recorded for the removed stage-2
-skip variant."""
+rules the schedule out as the cause of the GPU fault that a first form of the
+stage-2 skip (a flag bit or-ed into a tile's count) once caused in the batched
+autotune; the library's form writes a sentinel that stage 2 overwrites, and relies
+on every tile being visited by both stages, which is what this checks."""
 import numpy as np
 import pytest
 
