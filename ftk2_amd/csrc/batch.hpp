@@ -308,7 +308,7 @@ int run_batch(BatchArgs& A, std::vector<ftk2h_result*>& outs) {
       if constexpr (NV > 1) {
         if (nc) {
           const uint64_t g = (nc * (NV - 1) + BLOCK - 1) / BLOCK;
-          k_link_pairs<<<dim3((unsigned)std::min<uint64_t>(g, LINK_GRID)), dim3(BLOCK), 0, st>>>(parent, xedges, xcount);
+          k_link_pairs<NV != 2><<<dim3((unsigned)std::min<uint64_t>(g, LINK_GRID)), dim3(BLOCK), 0, st>>>(parent, xedges, xcount);
           HIP_TRY(hipGetLastError());
           kb_localize_cells<NV><<<dim3((unsigned)((nc + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st>>>(
               cells, cell_base, node_base, nj, nc);

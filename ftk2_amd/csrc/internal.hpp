@@ -528,7 +528,7 @@ int run_pipeline(RunArgs& A) {
     if (nc) {
       // at most nc * (NV - 1) pairs; threads beyond the device-side count exit
       const uint64_t g = (nc * (NV - 1) + BLOCK - 1) / BLOCK;
-      k_link_pairs<<<dim3((unsigned)std::min<uint64_t>(g, LINK_GRID)), dim3(BLOCK), 0, st>>>(parent, P.xedges, P.xcount);
+      k_link_pairs<NV != 2><<<dim3((unsigned)std::min<uint64_t>(g, LINK_GRID)), dim3(BLOCK), 0, st>>>(parent, P.xedges, P.xcount);
       HIP_TRY(hipGetLastError());
     }
   }

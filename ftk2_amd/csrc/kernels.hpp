@@ -1741,24 +1741,31 @@ __global__ __launch_bounds__(BLOCK) void k_emit(KParams P) {
   emit_tile<KIND, D, M, T>(P, blockIdx.x);
 }
 
-// union-find hooking (larger root under smaller) over every cell
+// union-find hooking (larger root under smaller).  COMPRESS: finds shorten the
+// paths they walk (path splitting).  Cross-tile linking of star edges runs without
+// it: their forest is shallow after the workgroup unions, and the stores cost more
+// than they save (256^3 x 4 CP3D step: linking 1.95 -> 1.79 ms, the final compress
+// pass 0.79 -> 0.83 ms); the denser marching cells need it (contour 384^3 x 4
+// linking 9.0 -> 11.6 ms without), and so do unions straight over a slab's cells.
 __device__ __forceinline__ uint32_t uf_load(uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+template <bool COMPRESS>
 __device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
   uint32_t p = uf_load(parent + x);
   while (p != x) {
     const uint32_t gp = uf_load(parent + p);
-    if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (COMPRESS && gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     x = p;
     p = gp;
   }
   return x;
 }
+template <bool COMPRESS>
 __device__ __forceinline__ void uf_unite(uint32_t* parent, uint32_t a, uint32_t b) {
   while (true) {
-    a = uf_find(parent, a);
-    b = uf_find(parent, b);
+    a = uf_find<COMPRESS>(parent, a);
+    b = uf_find<COMPRESS>(parent, b);
     if (a == b) return;
     if (a > b) {
       const uint32_t t = a;
@@ -1775,11 +1782,11 @@ __device__ __forceinline__ void uf_unite(uint32_t* parent, uint32_t a, uint32_t 
 
 // Grid-stride over the *count pairs (the count is only known on the device, so the
 // grid is a fixed persistent size instead of one thread per possible pair).
-static __global__ __launch_bounds__(BLOCK) void k_link_pairs(uint32_t* parent, const uint32_t* pairs,
-                                                      const uint32_t* count) {
+template <bool COMPRESS>
+__global__ __launch_bounds__(BLOCK) void k_link_pairs(uint32_t* parent, const uint32_t* pairs, const uint32_t* count) {
   const uint64_t n = *count;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
-    uf_unite(parent, pairs[2 * i], pairs[2 * i + 1]);
+    uf_unite<COMPRESS>(parent, pairs[2 * i], pairs[2 * i + 1]);
 }
 constexpr unsigned LINK_GRID = 256 * 16;  // workgroups of the persistent link kernel
 

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(BLOCK) void k_remap_cells(const uint32_t* cells, ui
   for (int q = 0; q < NV; ++q) out[i * NV + q] = c[q];
   if (!ok) return;  // reported by ftk2h_stream_finish; indices may lie beyond the parents
 #pragma unroll
-  for (int q = 1; q < NV; ++q) uf_unite(parent, c[0] - shift, c[q] - shift);
+  for (int q = 1; q < NV; ++q) uf_unite<true>(parent, c[0] - shift, c[q] - shift);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_iota(uint32_t* p, uint32_t base, uint64_t n) {
