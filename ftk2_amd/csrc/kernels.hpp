@@ -222,17 +222,20 @@ __device__ __forceinline__ uint32_t packed_chain(int t) {
 struct CornerTab {
   int64_t coff[3][16];
   int64_t soff[16];
+  int64_t loff[16];
   uint32_t pc[64];
 };
 template <int D, int M>
 __device__ __forceinline__ void load_corner_tab(const KParams& P, CornerTab* tab) {
-  for (int i = threadIdx.x; i < 48 + 16 + 64; i += BLOCK) {
+  for (int i = threadIdx.x; i < 48 + 16 + 16 + 64; i += BLOCK) {
     if (i < 48)
       tab->coff[i / 16][i % 16] = P.coff[i / 16][i % 16];
     else if (i < 64)
       tab->soff[i - 48] = P.soff[i - 48];
+    else if (i < 80)
+      tab->loff[i - 64] = P.loff[i - 64];
     else
-      tab->pc[i - 64] = packed_chain<D, M>(i - 64);
+      tab->pc[i - 80] = packed_chain<D, M>(i - 80);
   }
 }
 template <class T>
@@ -1477,6 +1480,154 @@ __device__ __forceinline__ void sort_cells(Ptr seg, uint32_t n) {
   }
 }
 
+// Marching cells as workgroup items (contour, d = 4 / 3).  Near a level set only a
+// few of a workgroup's vertices have cells, and each of them only in a few of its
+// pentatopes (tetrahedra for d = 3); looping over every pentatope of every vertex
+// runs both cell shapes for the whole wave whenever one lane has them.  Instead
+// every vertex publishes the cells it has (march_masks) and the lanes take one
+// (vertex, pentatope) item each (marching_item).  The cells are those of cells_of's
+// marching branch; their order inside a hypercube is set afterwards by sorting.
+
+// Pentatope p's cell vertices (chain offsets), four bits each.
+template <int D>
+struct MarchPack {
+  static constexpr uint32_t of(int p) {
+    uint32_t r = 0;
+    for (int q = 0; q <= D; ++q) r |= (uint32_t)CF<D, 1>.cell[p][q] << (4 * q);
+    return r;
+  }
+  // the base vertex's edge types whose nodes decide pentatope p's shape
+  static constexpr uint64_t rel(int p) {
+    uint64_t r = 0;
+    for (int q = 1; q <= D; ++q) r |= 1ull << (CF<D, 1>.cell[p][q] - 1);
+    return r;
+  }
+};
+
+// act: pentatopes with cells; big: those with the larger shape (three cells for
+// d = 4, two for d = 3).  mk0: the base vertex's node mask.
+template <int D, int CK>
+__device__ __forceinline__ void march_masks(uint64_t mk0, uint32_t* act, uint32_t* big) {
+  constexpr int NPC = CellFaceTable<D, 1>::NP;
+  uint32_t a = 0, b = 0;
+  sfor<0, NPC>([&](auto pp) {
+    constexpr int p = decltype(pp)::value;
+    const int g1 = __popcll(mk0 & MarchPack<D>::rel(p));
+    if (g1 != 0) a |= 1u << p;
+    if (CK == 3 ? (g1 == 2 || g1 == 3) : g1 == 2) b |= 1u << p;
+  });
+  *act = a;
+  *big = b;
+}
+
+// The cells of marching pentatope / tetrahedron p of the hypercube based at local
+// vertex u, each sorted, written to dst (1 or 3 cells for d = 4, 1 or 2 for d = 3).
+// Edge (i, j) of the cell (positions i < j) carries the node of type
+// c_i ^ c_j - 1 owned by vertex u + c_i; vertex pairs are compile-time, so no
+// register array is indexed at run time.
+template <int D, int CK>
+__device__ __forceinline__ void marching_item(const KParams& P, const CornerTab& tab, int64_t u, int p,
+                                              uint32_t* dst) {
+  constexpr int NQ = D + 1;
+  constexpr int NPC = CellFaceTable<D, 1>::NP;
+  constexpr int NV = CK == 3 ? 4 : 3;
+  uint32_t pk = 0;
+  sfor<0, NPC>([&](auto pp) {
+    constexpr int pi = decltype(pp)::value;
+    if (p == pi) pk = MarchPack<D>::of(pi);
+  });
+  int ce[NQ];
+  uint64_t mk[NQ - 1];
+  uint32_t no[NQ - 1];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) ce[q] = (int)((pk >> (4 * q)) & 15u);
+#pragma unroll
+  for (int q = 0; q < NQ - 1; ++q) {  // the last vertex owns no edge of the cell
+    const int64_t w = u + tab.loff[ce[q]];
+    mk[q] = gptr(P.mask)[w];
+    no[q] = gptr(P.node_off)[w];
+  }
+  int rel[NQ];
+  int g1 = 0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    rel[q] = q == 0 ? 0 : (int)((mk[0] >> (ce[q] - 1)) & 1);
+    g1 += rel[q];
+  }
+  auto E = [&](int i, int j) -> uint32_t {  // i < j, both compile-time after unrolling
+    return no[i] + (uint32_t)__popcll(mk[i] & ((1ull << ((ce[i] ^ ce[j]) - 1)) - 1ull));
+  };
+  auto put = [&](int k, uint32_t* c) {
+    sort_small<NV>(c);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) dst[k * NV + i] = c[i];
+  };
+  const bool single = g1 == 1 || g1 == NQ - 1;
+  if (single || (CK == 4 && g1 == 2)) {
+    // the D active edges (one vertex alone: its edges; d = 3, two and two: the four
+    // edges between the pairs), in any order
+    uint32_t c[4] = {0, 0, 0, 0};
+    int k = 0;
+    sfor<0, NQ>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      sfor<i + 1, NQ>([&](auto jj) {
+        constexpr int j = decltype(jj)::value;
+        if (rel[i] != rel[j]) {
+          const uint32_t e = E(i, j);
+#pragma unroll
+          for (int s = 0; s < 4; ++s)
+            if (k == s) c[s] = e;
+          k++;
+        }
+      });
+    });
+    if (single) {
+      put(0, c);
+    } else {
+      sort_small<4>(c);
+      uint32_t c0[3] = {c[0], c[1], c[2]};
+      uint32_t c1[3] = {c[0], c[2], c[3]};
+      put(0, c0);
+      put(1, c1);
+    }
+  } else if constexpr (CK == 3) {
+    // two against three: T[3 * (rank of a in A) + (rank of b in B)] = node of edge
+    // (a, b), A the pair, ranks by position; the prism splits along its staircase
+    const int relA = g1 == 2 ? 1 : 0;
+    int rA[NQ], rB[NQ];
+    int na = 0, nb = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      rA[q] = na;
+      rB[q] = nb;
+      if (rel[q] == relA)
+        na++;
+      else
+        nb++;
+    }
+    uint32_t T[6] = {0, 0, 0, 0, 0, 0};
+    sfor<0, NQ>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      sfor<i + 1, NQ>([&](auto jj) {
+        constexpr int j = decltype(jj)::value;
+        if (rel[i] != rel[j]) {
+          const int slot = rel[i] == relA ? 3 * rA[i] + rB[j] : 3 * rA[j] + rB[i];
+          const uint32_t e = E(i, j);
+#pragma unroll
+          for (int s = 0; s < 6; ++s)
+            if (slot == s) T[s] = e;
+        }
+      });
+    });
+    uint32_t c0[4] = {T[0], T[1], T[2], T[5]};
+    uint32_t c1[4] = {T[0], T[1], T[4], T[5]};
+    uint32_t c2[4] = {T[0], T[3], T[4], T[5]};
+    put(0, c0);
+    put(1, c1);
+    put(2, c2);
+  }
+}
+
 // Position of staged cell c inside its vertex's segment [s, e) of the staging
 // array: the number of the segment's cells ordered before it (lexicographic, equal
 // cells in staging order).  One thread per cell, so a vertex with many cells costs
@@ -1551,7 +1702,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   constexpr int NV = CellKind<KIND, D, M>::verts;
   // a tile without nodes has no cells (k_offsets left its counts unwritten)
   if (gptr(P.bsum)[blk + 1] == gptr(P.bsum)[blk]) return;
-  __shared__ uint16_t nq[EMIT_NQ];  // node queue, then the workgroup union-find
+  __shared__ __align__(16) uint16_t nq[EMIT_NQ];  // node queue, marching items, then the union-find
   __shared__ VtxRef refs[BLOCK];
   __shared__ uint32_t stage[EMIT_STAGE_WORDS];
   __shared__ CornerTab tab;
@@ -1637,14 +1788,33 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       constexpr bool OWNERS = NV == 2;
       static_assert(sizeof(refs) >= EMIT_STAGE_WORDS / 2, "cell owners do not fit");
       uint8_t* cown = reinterpret_cast<uint8_t*>(refs);
-      if (nc) {
-        cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
-        if constexpr (OWNERS)
-          for (uint32_t c = ex; c < ex + nc; ++c) cown[c] = (uint8_t)threadIdx.x;
-      }
+      constexpr int CKV = CellKind<KIND, D, M>::value;
       segx[threadIdx.x] = ex;
       if (threadIdx.x == 0) segx[BLOCK] = ctot;
-      __syncthreads();
+      if constexpr (CKV == 3 || CKV == 4) {
+        // marching cells as (vertex, pentatope) items; the node queue and the vertex
+        // references of the node phase are free
+        static_assert(sizeof(ItemLds<0>) <= sizeof(nq) && sizeof(refs) >= 2 * BLOCK * 4, "marching items do not fit");
+        uint32_t act = 0, big = 0;
+        if (nc) march_masks<D, CKV>(m, &act, &big);
+        uint32_t* actl = reinterpret_cast<uint32_t*>(refs);
+        uint32_t* bigl = actl + BLOCK;
+        actl[threadIdx.x] = act;
+        bigl[threadIdx.x] = big;
+        run_items(*reinterpret_cast<ItemLds<0>*>(nq), (uint64_t)act, [&](int owner, int p) {
+          const uint32_t below = (1u << p) - 1u;
+          const uint32_t k = segx[owner] + (uint32_t)__popc(actl[owner] & below) +
+                             (CKV == 3 ? 2u : 1u) * (uint32_t)__popc(bigl[owner] & below);
+          marching_item<D, CKV>(P, tab, blk * BLOCK + owner, p, stage + k * NV);
+        });
+      } else {
+        if (nc) {
+          cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
+          if constexpr (OWNERS)
+            for (uint32_t c = ex; c < ex + nc; ++c) cown[c] = (uint8_t)threadIdx.x;
+        }
+        __syncthreads();
+      }
       // each cell goes straight to its sorted place in the workgroup's output range
       auto dst = gptr(P.cells) + cbase * NV;
       for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
