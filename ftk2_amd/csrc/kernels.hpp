@@ -1712,12 +1712,18 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   Vtx x;
   uint64_t m = 0;
   bool hc = false;
+  // every per-vertex and per-tile input of the tile is requested here, before the
+  // first barrier (one memory round trip instead of one per phase)
+  uint32_t pre_off = 0, pre_nc = 0;
   if (v < P.nvert) {
     x = make_vtx<D>(P, v);
     m = gptr(P.mask)[v];
+    pre_off = gptr(P.node_off)[v];
+    if constexpr (NV > 0) pre_nc = gptr(P.ccount)[v];
     hc = is_hc_base<D>(P, x.l);
     refs[threadIdx.x] = x.r;
   }
+  const uint64_t pre_cbase = NV > 0 ? gptr(P.ebsum)[blk] : 0;
   // ---- nodes
   const uint32_t nbase = gptr(P.bsum)[blk];
   const uint32_t ntot = gptr(P.bsum)[blk + 1] - nbase;
@@ -1733,7 +1739,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   };
   if (ntot <= P.nq_cap) {
     if (m) {
-      uint32_t pos = (v < P.nvert ? gptr(P.node_off)[v] : 0) - nbase;
+      uint32_t pos = pre_off - nbase;
       for (uint64_t mm = m; mm; mm &= mm - 1) nq[pos++] = (uint16_t)(threadIdx.x | (__builtin_ctzll(mm) << 8));
     }
     __syncthreads();
@@ -1762,7 +1768,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       }
     }
   } else if (m) {  // more nodes than the queue holds: each thread writes its own
-    uint32_t pos = gptr(P.node_off)[v];
+    uint32_t pos = pre_off;
     for (uint64_t mm = m; mm; mm &= mm - 1) {
       const int t = __builtin_ctzll(mm);
       T vals[M + 1][M];
@@ -1774,10 +1780,10 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   // ---- cells
   if constexpr (NV > 0) {
     // the count k_offsets made for this hypercube
-    const uint32_t nc = (v < P.nvert && hc) ? (uint32_t)gptr(P.ccount)[v] : 0u;
+    const uint32_t nc = hc ? pre_nc : 0u;
     uint32_t ctot;
     const uint32_t ex = block_excl_scan(nc, &ctot);
-    const uint64_t cbase = gptr(P.ebsum)[blk];
+    const uint64_t cbase = pre_cbase;
     if (ctot * NV <= P.stage_cap) {
       __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
       // Star edges (two or three per vertex): the thread whose segment holds each
