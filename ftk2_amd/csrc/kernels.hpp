@@ -1696,6 +1696,102 @@ __device__ __forceinline__ void lds_unite(uint32_t* p, uint32_t a, uint32_t b) {
 }
 constexpr int EMIT_STAGE_WORDS = 4096;  // staged cell indices per workgroup
 
+// Workgroup union-find of cells with three or four nodes (ntot <= EMIT_LUF nodes,
+// lpar in the node queue's memory; xh: EMIT_XH slots of free LDS).  Each cell's nodes
+// inside the workgroup are united in LDS through its first such node; every node
+// outside it is paired with that node's root (with the cell's first node when none
+// is inside) for the global union-find.  A node lies in many cells, so the pairs
+// repeat: an LDS hash set keeps one of each (a pair that finds no slot is kept, a
+// repeated union being harmless).  Ends with the pairs written.
+constexpr int EMIT_XH = 1024;
+template <int NV>
+__device__ __forceinline__ void emit_link_cells(const KParams& P, uint16_t* nq, void* free_lds, const uint32_t* stage,
+                                                uint32_t nbase, uint32_t ntot, uint32_t ctot) {
+  static_assert(NV >= 3 && NV <= 4, "cells of three or four nodes");
+  static_assert((EMIT_STAGE_WORDS / NV + BLOCK - 1) / BLOCK * NV <= 32, "a thread's pair bits exceed 32");
+  constexpr unsigned long long EMPTY = ~0ull;
+  uint32_t* lpar = reinterpret_cast<uint32_t*>(nq);
+  unsigned long long* xh = reinterpret_cast<unsigned long long*>(free_lds);
+  for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) lpar[i] = i;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)EMIT_XH; i += BLOCK) xh[i] = EMPTY;
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
+    uint32_t f = ~0u;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const uint32_t n = stage[c * NV + q] - nbase;
+      if (n < ntot) {
+        if (f == ~0u)
+          f = n;
+        else
+          lds_unite(lpar, f, n);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) gptr(P.parent)[nbase + i] = nbase + lds_find(lpar, i);
+  // the pairs of cell c: anchor (root of its first inside node, else its first node)
+  // with each outside node other than the anchor; bit c_local * NV + q of `own`
+  // marks the pairs this thread keeps
+  auto anchor_of = [&](uint32_t c, bool* inside) {
+    uint32_t f = ~0u;
+#pragma unroll
+    for (int q = NV - 1; q >= 0; --q) {
+      const uint32_t n = stage[c * NV + q] - nbase;
+      if (n < ntot) f = n;
+    }
+    *inside = f != ~0u;
+    return f != ~0u ? nbase + lds_find(lpar, f) : stage[c * NV];
+  };
+  uint32_t own = 0, nx = 0;
+  int bit = 0;
+  for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK, bit += NV) {
+    bool inside;
+    const uint32_t anc = anchor_of(c, &inside);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const uint32_t g = stage[c * NV + q];
+      if (g - nbase < ntot || (!inside && q == 0)) continue;
+      const unsigned long long key =
+          ((unsigned long long)min(anc, g) << 32) | (unsigned long long)max(anc, g);
+      const uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 54);  // 10 bits: EMIT_XH slots
+      bool keep = true;
+#pragma unroll
+      for (int probe = 0; probe < 8; ++probe) {
+        const unsigned long long old = atomicCAS(&xh[(h + probe) & (EMIT_XH - 1)], EMPTY, key);
+        if (old == EMPTY) break;
+        if (old == key) {
+          keep = false;
+          break;
+        }
+      }
+      if (keep) {
+        own |= 1u << (bit + q);
+        nx++;
+      }
+    }
+  }
+  uint32_t xtot;
+  uint32_t xo = block_excl_scan(nx, &xtot);
+  __shared__ uint32_t xbase;
+  if (threadIdx.x == 0) xbase = xtot ? atomicAdd(P.xcount, xtot) : 0;
+  __syncthreads();
+  xo += xbase;
+  bit = 0;
+  for (uint32_t c = threadIdx.x; c < ctot && own; c += BLOCK, bit += NV) {
+    if (((own >> bit) & ((1u << NV) - 1u)) == 0) continue;
+    bool inside;
+    const uint32_t anc = anchor_of(c, &inside);
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      if ((own >> (bit + q)) & 1u) {
+        gptr(P.xedges)[2 * (uint64_t)xo] = anc;
+        gptr(P.xedges)[2 * (uint64_t)xo + 1] = stage[c * NV + q];
+        xo++;
+      }
+  }
+}
+
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
   constexpr int NTY = Chains<D, M>::N;
@@ -1845,7 +1941,11 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       }
       // Union the cells whose nodes all belong to this workgroup in LDS and start
       // the global union-find from those roots (roots stay component minima).
-      if (ntot <= EMIT_LUF) {
+      if (NV >= 3 && ntot <= EMIT_LUF) {
+        static_assert(sizeof(refs) >= EMIT_XH * 8, "the pair set does not fit");
+        emit_link_cells<NV < 3 ? 3 : NV>(P, nq, refs, stage, nbase, ntot, ctot);
+        linked = true;
+      } else if (ntot <= EMIT_LUF) {
         uint32_t* lpar = reinterpret_cast<uint32_t*>(nq);  // the node queue is done
         for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) lpar[i] = i;
         __syncthreads();
