@@ -1642,18 +1642,16 @@ __device__ __forceinline__ uint32_t cell_rank(const uint32_t* stage, uint32_t s,
       r += (kj < kc || (kj == kc && j < c)) ? 1u : 0u;
     }
   } else {
-    uint32_t a[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) a[q] = stage[c * NV + q];
+    // two 64-bit keys: the first two indices, then the rest
+    auto hi = [&](uint32_t j) { return ((uint64_t)stage[j * NV] << 32) | stage[j * NV + 1]; };
+    auto lo = [&](uint32_t j) {
+      return NV == 4 ? ((uint64_t)stage[j * NV + 2] << 32) | stage[j * NV + NV - 1] : (uint64_t)stage[j * NV + 2];
+    };
+    const uint64_t ch = hi(c), cl = lo(c);
+#pragma unroll 4
     for (uint32_t j = s; j < e; ++j) {
-      bool lt = false, eq = true;
-#pragma unroll
-      for (int q = 0; q < NV; ++q) {
-        const uint32_t b = stage[j * NV + q];
-        lt = lt || (eq && b < a[q]);
-        eq = eq && b == a[q];
-      }
-      r += (lt || (eq && j < c)) ? 1u : 0u;
+      const uint64_t jh = hi(j), jl = lo(j);
+      r += (jh < ch || (jh == ch && (jl < cl || (jl == cl && j < c)))) ? 1u : 0u;
     }
   }
   return r;
@@ -1697,21 +1695,22 @@ __device__ __forceinline__ void lds_unite(uint32_t* p, uint32_t a, uint32_t b) {
 constexpr int EMIT_STAGE_WORDS = 4096;  // staged cell indices per workgroup
 
 // Workgroup union-find of cells with three or four nodes (ntot <= EMIT_LUF nodes,
-// lpar in the node queue's memory; xh: EMIT_XH slots of free LDS).  Each cell's nodes
-// inside the workgroup are united in LDS through its first such node; every node
-// outside it is paired with that node's root (with the cell's first node when none
-// is inside) for the global union-find.  A node lies in many cells, so the pairs
-// repeat: an LDS hash set keeps one of each (a pair that finds no slot is kept, a
-// repeated union being harmless).  Ends with the pairs written.
+// lpar in the node queue's memory; xh: EMIT_XH slots of free LDS; the cells in LDS
+// or, for a workgroup whose cells overflow the staging array, in global memory).
+// Each cell's nodes inside the workgroup are united in LDS through its first such
+// node; every node outside it is paired with that node's root (with the cell's
+// first node when none is inside) for the global union-find.  A node lies in many
+// cells, so the pairs repeat: an LDS hash set keeps one of each (a pair that finds
+// no slot is kept, a repeated union being harmless).  Ends with the pairs written.
 constexpr int EMIT_XH = 1024;
-template <int NV>
-__device__ __forceinline__ void emit_link_cells(const KParams& P, uint16_t* nq, void* free_lds, const uint32_t* stage,
+template <int NV, class Src>
+__device__ __forceinline__ void emit_link_cells(const KParams& P, uint16_t* nq, void* free_lds, Src cl,
                                                 uint32_t nbase, uint32_t ntot, uint32_t ctot) {
   static_assert(NV >= 3 && NV <= 4, "cells of three or four nodes");
-  static_assert((EMIT_STAGE_WORDS / NV + BLOCK - 1) / BLOCK * NV <= 32, "a thread's pair bits exceed 32");
   constexpr unsigned long long EMPTY = ~0ull;
   uint32_t* lpar = reinterpret_cast<uint32_t*>(nq);
   unsigned long long* xh = reinterpret_cast<unsigned long long*>(free_lds);
+  __syncthreads();  // the caller's last reads of that LDS (the cell owners) are done
   for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) lpar[i] = i;
   for (uint32_t i = threadIdx.x; i < (uint32_t)EMIT_XH; i += BLOCK) xh[i] = EMPTY;
   __syncthreads();
@@ -1719,7 +1718,7 @@ __device__ __forceinline__ void emit_link_cells(const KParams& P, uint16_t* nq, 
     uint32_t f = ~0u;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      const uint32_t n = stage[c * NV + q] - nbase;
+      const uint32_t n = cl[c * NV + q] - nbase;
       if (n < ntot) {
         if (f == ~0u)
           f = n;
@@ -1731,64 +1730,69 @@ __device__ __forceinline__ void emit_link_cells(const KParams& P, uint16_t* nq, 
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < ntot; i += BLOCK) gptr(P.parent)[nbase + i] = nbase + lds_find(lpar, i);
   // the pairs of cell c: anchor (root of its first inside node, else its first node)
-  // with each outside node other than the anchor; bit c_local * NV + q of `own`
-  // marks the pairs this thread keeps
+  // with each outside node other than the anchor
   auto anchor_of = [&](uint32_t c, bool* inside) {
     uint32_t f = ~0u;
 #pragma unroll
     for (int q = NV - 1; q >= 0; --q) {
-      const uint32_t n = stage[c * NV + q] - nbase;
+      const uint32_t n = cl[c * NV + q] - nbase;
       if (n < ntot) f = n;
     }
     *inside = f != ~0u;
-    return f != ~0u ? nbase + lds_find(lpar, f) : stage[c * NV];
+    return f != ~0u ? nbase + lds_find(lpar, f) : (uint32_t)cl[c * NV];
   };
-  uint32_t own = 0, nx = 0;
-  int bit = 0;
-  for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK, bit += NV) {
-    bool inside;
-    const uint32_t anc = anchor_of(c, &inside);
+  // rounds of CPR cells per thread: bit (round cell) * NV + q of `own` marks the
+  // pairs this thread keeps
+  constexpr uint32_t CPR = 32 / NV;
+  __shared__ uint32_t xbase;
+  for (uint32_t base = 0; base < ctot; base += CPR * BLOCK) {
+    const uint32_t end = min(ctot, base + CPR * BLOCK);
+    uint32_t own = 0, nx = 0;
+    int bit = 0;
+    for (uint32_t c = base + threadIdx.x; c < end; c += BLOCK, bit += NV) {
+      bool inside;
+      const uint32_t anc = anchor_of(c, &inside);
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const uint32_t g = stage[c * NV + q];
-      if (g - nbase < ntot || (!inside && q == 0)) continue;
-      const unsigned long long key =
-          ((unsigned long long)min(anc, g) << 32) | (unsigned long long)max(anc, g);
-      const uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 54);  // 10 bits: EMIT_XH slots
-      bool keep = true;
+      for (int q = 0; q < NV; ++q) {
+        const uint32_t g = cl[c * NV + q];
+        if (g - nbase < ntot || (!inside && q == 0)) continue;
+        const unsigned long long key =
+            ((unsigned long long)min(anc, g) << 32) | (unsigned long long)max(anc, g);
+        const uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 54);  // 10 bits: EMIT_XH slots
+        bool keep = true;
 #pragma unroll
-      for (int probe = 0; probe < 8; ++probe) {
-        const unsigned long long old = atomicCAS(&xh[(h + probe) & (EMIT_XH - 1)], EMPTY, key);
-        if (old == EMPTY) break;
-        if (old == key) {
-          keep = false;
-          break;
+        for (int probe = 0; probe < 8; ++probe) {
+          const unsigned long long old = atomicCAS(&xh[(h + probe) & (EMIT_XH - 1)], EMPTY, key);
+          if (old == EMPTY) break;
+          if (old == key) {
+            keep = false;
+            break;
+          }
+        }
+        if (keep) {
+          own |= 1u << (bit + q);
+          nx++;
         }
       }
-      if (keep) {
-        own |= 1u << (bit + q);
-        nx++;
-      }
     }
-  }
-  uint32_t xtot;
-  uint32_t xo = block_excl_scan(nx, &xtot);
-  __shared__ uint32_t xbase;
-  if (threadIdx.x == 0) xbase = xtot ? atomicAdd(P.xcount, xtot) : 0;
-  __syncthreads();
-  xo += xbase;
-  bit = 0;
-  for (uint32_t c = threadIdx.x; c < ctot && own; c += BLOCK, bit += NV) {
-    if (((own >> bit) & ((1u << NV) - 1u)) == 0) continue;
-    bool inside;
-    const uint32_t anc = anchor_of(c, &inside);
+    uint32_t xtot;
+    uint32_t xo = block_excl_scan(nx, &xtot);
+    if (threadIdx.x == 0) xbase = xtot ? atomicAdd(P.xcount, xtot) : 0;
+    __syncthreads();
+    xo += xbase;
+    bit = 0;
+    for (uint32_t c = base + threadIdx.x; c < end && own; c += BLOCK, bit += NV) {
+      if (((own >> bit) & ((1u << NV) - 1u)) == 0) continue;
+      bool inside;
+      const uint32_t anc = anchor_of(c, &inside);
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-      if ((own >> (bit + q)) & 1u) {
-        gptr(P.xedges)[2 * (uint64_t)xo] = anc;
-        gptr(P.xedges)[2 * (uint64_t)xo + 1] = stage[c * NV + q];
-        xo++;
-      }
+      for (int q = 0; q < NV; ++q)
+        if ((own >> (bit + q)) & 1u) {
+          gptr(P.xedges)[2 * (uint64_t)xo] = anc;
+          gptr(P.xedges)[2 * (uint64_t)xo + 1] = cl[c * NV + q];
+          xo++;
+        }
+    }
   }
 }
 
@@ -1882,15 +1886,18 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
     const uint64_t cbase = pre_cbase;
     if (ctot * NV <= P.stage_cap) {
       __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
-      // Star edges (two or three per vertex): the thread whose segment holds each
-      // cell (at most EMIT_STAGE_WORDS / 2 cells) in the LDS of the node phase's
-      // vertex references, which are no longer read.  Marching cells come many per
-      // hypercube, where one thread's serial byte stores cost more than a binary
-      // search per cell (contour emit 45.5 -> 66.7 ms with the stores).
-      constexpr bool OWNERS = NV == 2;
-      static_assert(sizeof(refs) >= EMIT_STAGE_WORDS / 2, "cell owners do not fit");
-      uint8_t* cown = reinterpret_cast<uint8_t*>(refs);
+      // The thread whose segment holds each cell (at most EMIT_STAGE_WORDS / 2
+      // cells), in the LDS of the node phase's vertex references, which are no
+      // longer read, where its writes are cheap: star edges (two or three per
+      // vertex) by their vertex's thread, marching cells (one to three per item) by
+      // the item's lane.  Star-fan cells come many per vertex, where one thread's
+      // serial byte stores cost more than a binary search of segx per cell (so did
+      // marching cells written per vertex: contour emit 45.5 -> 66.7 ms).
       constexpr int CKV = CellKind<KIND, D, M>::value;
+      constexpr bool OWNERS = NV == 2 || CKV == 3 || CKV == 4;
+      constexpr int OWN_AT = (CKV == 3 || CKV == 4) ? 2 * BLOCK * 4 : 0;  // after the items' masks
+      static_assert(sizeof(refs) >= OWN_AT + EMIT_STAGE_WORDS / 2, "cell owners do not fit");
+      uint8_t* cown = reinterpret_cast<uint8_t*>(refs) + OWN_AT;
       segx[threadIdx.x] = ex;
       if (threadIdx.x == 0) segx[BLOCK] = ctot;
       if constexpr (CKV == 3 || CKV == 4) {
@@ -1908,6 +1915,8 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
           const uint32_t k = segx[owner] + (uint32_t)__popc(actl[owner] & below) +
                              (CKV == 3 ? 2u : 1u) * (uint32_t)__popc(bigl[owner] & below);
           marching_item<D, CKV>(P, tab, blk * BLOCK + owner, p, stage + k * NV);
+          const uint32_t ncell = ((bigl[owner] >> p) & 1u) ? (CKV == 3 ? 3u : 2u) : 1u;
+          for (uint32_t i = 0; i < ncell; ++i) cown[k + i] = (uint8_t)owner;
         });
       } else {
         if (nc) {
@@ -1943,7 +1952,7 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       // the global union-find from those roots (roots stay component minima).
       if (NV >= 3 && ntot <= EMIT_LUF) {
         static_assert(sizeof(refs) >= EMIT_XH * 8, "the pair set does not fit");
-        emit_link_cells<NV < 3 ? 3 : NV>(P, nq, refs, stage, nbase, ntot, ctot);
+        emit_link_cells<NV < 3 ? 3 : NV>(P, nq, refs, static_cast<const uint32_t*>(stage), nbase, ntot, ctot);
         linked = true;
       } else if (ntot <= EMIT_LUF) {
         uint32_t* lpar = reinterpret_cast<uint32_t*>(nq);  // the node queue is done
@@ -1988,10 +1997,22 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
         }
         linked = true;
       }
-    } else if (nc) {
-      auto seg = gptr(P.cells) + (cbase + ex) * NV;
-      cells_of<KIND, D, M>(P, x, 1, seg);
-      sort_cells<NV>(seg, nc);
+    } else {
+      if (nc) {
+        auto seg = gptr(P.cells) + (cbase + ex) * NV;
+        cells_of<KIND, D, M>(P, x, 1, seg);
+        sort_cells<NV>(seg, nc);
+      }
+      if (NV >= 3 && ntot <= EMIT_LUF) {
+        // more cells than the staging array holds: the same union-find over the
+        // cells in global memory (each thread's own writes are visible to the
+        // workgroup after the barrier)
+        static_assert(sizeof(refs) >= EMIT_XH * 8, "the pair set does not fit");
+        __syncthreads();
+        const FTK2H_AS1 uint32_t* src = gptr(P.cells) + cbase * NV;
+        emit_link_cells<NV < 3 ? 3 : NV>(P, nq, refs, src, nbase, ntot, ctot);
+        linked = true;
+      }
     }
     if (!linked) {
       // every edge of this workgroup goes to the global union-find
