@@ -1521,13 +1521,13 @@ __device__ __forceinline__ void march_masks(uint64_t mk0, uint32_t* act, uint32_
 }
 
 // The cells of marching pentatope / tetrahedron p of the hypercube based at local
-// vertex u, each sorted, written to dst (1 or 3 cells for d = 4, 1 or 2 for d = 3).
+// vertex u, each sorted, written to dst (LDS or global memory; 1 or 3 cells for
+// d = 4, 1 or 2 for d = 3).
 // Edge (i, j) of the cell (positions i < j) carries the node of type
 // c_i ^ c_j - 1 owned by vertex u + c_i; vertex pairs are compile-time, so no
 // register array is indexed at run time.
-template <int D, int CK>
-__device__ __forceinline__ void marching_item(const KParams& P, const CornerTab& tab, int64_t u, int p,
-                                              uint32_t* dst) {
+template <int D, int CK, class Dst>
+__device__ __forceinline__ void marching_item(const KParams& P, const CornerTab& tab, int64_t u, int p, Dst dst) {
   constexpr int NQ = D + 1;
   constexpr int NPC = CellFaceTable<D, 1>::NP;
   constexpr int NV = CK == 3 ? 4 : 3;
@@ -1884,8 +1884,34 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
     uint32_t ctot;
     const uint32_t ex = block_excl_scan(nc, &ctot);
     const uint64_t cbase = pre_cbase;
+    __shared__ uint32_t segx[BLOCK + 1];  // offset of every thread's segment of cells
+    constexpr int CKV = CellKind<KIND, D, M>::value;
+    // marching cells as (vertex, pentatope) items written at cells + k * NV (k: the
+    // cell's offset in the workgroup), the owner thread of each cell recorded in
+    // cown when given; the node queue and the vertex references of the node phase
+    // are free.  Ends with a barrier.
+    auto march = [&](auto cells, uint8_t* cown) {
+      static_assert(sizeof(ItemLds<0>) <= sizeof(nq) && sizeof(refs) >= 2 * BLOCK * 4, "marching items do not fit");
+      uint32_t act = 0, big = 0;
+      if (nc) march_masks<D, CKV>(m, &act, &big);
+      uint32_t* actl = reinterpret_cast<uint32_t*>(refs);
+      uint32_t* bigl = actl + BLOCK;
+      actl[threadIdx.x] = act;
+      bigl[threadIdx.x] = big;
+      run_items(*reinterpret_cast<ItemLds<0>*>(nq), (uint64_t)act, [&](int owner, int p) {
+        const uint32_t below = (1u << p) - 1u;
+        const uint32_t k = segx[owner] + (uint32_t)__popc(actl[owner] & below) +
+                           (CKV == 3 ? 2u : 1u) * (uint32_t)__popc(bigl[owner] & below);
+        marching_item<D, CKV>(P, tab, blk * BLOCK + owner, p, cells + k * NV);
+        if (cown) {
+          const uint32_t ncell = ((bigl[owner] >> p) & 1u) ? (CKV == 3 ? 3u : 2u) : 1u;
+          for (uint32_t i = 0; i < ncell; ++i) cown[k + i] = (uint8_t)owner;
+        }
+      });
+    };
+    segx[threadIdx.x] = ex;
+    if (threadIdx.x == 0) segx[BLOCK] = ctot;
     if (ctot * NV <= P.stage_cap) {
-      __shared__ uint32_t segx[BLOCK + 1];  // staging offset of every thread's segment
       // The thread whose segment holds each cell (at most EMIT_STAGE_WORDS / 2
       // cells), in the LDS of the node phase's vertex references, which are no
       // longer read, where its writes are cheap: star edges (two or three per
@@ -1893,31 +1919,12 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       // the item's lane.  Star-fan cells come many per vertex, where one thread's
       // serial byte stores cost more than a binary search of segx per cell (so did
       // marching cells written per vertex: contour emit 45.5 -> 66.7 ms).
-      constexpr int CKV = CellKind<KIND, D, M>::value;
       constexpr bool OWNERS = NV == 2 || CKV == 3 || CKV == 4;
       constexpr int OWN_AT = (CKV == 3 || CKV == 4) ? 2 * BLOCK * 4 : 0;  // after the items' masks
       static_assert(sizeof(refs) >= OWN_AT + EMIT_STAGE_WORDS / 2, "cell owners do not fit");
       uint8_t* cown = reinterpret_cast<uint8_t*>(refs) + OWN_AT;
-      segx[threadIdx.x] = ex;
-      if (threadIdx.x == 0) segx[BLOCK] = ctot;
       if constexpr (CKV == 3 || CKV == 4) {
-        // marching cells as (vertex, pentatope) items; the node queue and the vertex
-        // references of the node phase are free
-        static_assert(sizeof(ItemLds<0>) <= sizeof(nq) && sizeof(refs) >= 2 * BLOCK * 4, "marching items do not fit");
-        uint32_t act = 0, big = 0;
-        if (nc) march_masks<D, CKV>(m, &act, &big);
-        uint32_t* actl = reinterpret_cast<uint32_t*>(refs);
-        uint32_t* bigl = actl + BLOCK;
-        actl[threadIdx.x] = act;
-        bigl[threadIdx.x] = big;
-        run_items(*reinterpret_cast<ItemLds<0>*>(nq), (uint64_t)act, [&](int owner, int p) {
-          const uint32_t below = (1u << p) - 1u;
-          const uint32_t k = segx[owner] + (uint32_t)__popc(actl[owner] & below) +
-                             (CKV == 3 ? 2u : 1u) * (uint32_t)__popc(bigl[owner] & below);
-          marching_item<D, CKV>(P, tab, blk * BLOCK + owner, p, stage + k * NV);
-          const uint32_t ncell = ((bigl[owner] >> p) & 1u) ? (CKV == 3 ? 3u : 2u) : 1u;
-          for (uint32_t i = 0; i < ncell; ++i) cown[k + i] = (uint8_t)owner;
-        });
+        march(stage, cown);
       } else {
         if (nc) {
           cells_of<KIND, D, M>(P, x, 1, stage + ex * NV);
@@ -1998,8 +2005,13 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
         linked = true;
       }
     } else {
-      if (nc) {
-        auto seg = gptr(P.cells) + (cbase + ex) * NV;
+      auto seg = gptr(P.cells) + (cbase + ex) * NV;
+      if constexpr (CKV == 3 || CKV == 4) {
+        // straight to the output range, then each vertex's thread sorts its segment
+        // (the items' writes are visible to the workgroup after their barrier)
+        march(gptr(P.cells) + cbase * NV, static_cast<uint8_t*>(nullptr));
+        if (nc) sort_cells<NV>(seg, nc);
+      } else if (nc) {
         cells_of<KIND, D, M>(P, x, 1, seg);
         sort_cells<NV>(seg, nc);
       }
