@@ -1631,9 +1631,11 @@ __device__ __forceinline__ void marching_item(const KParams& P, const CornerTab&
 // Position of staged cell c inside its vertex's segment [s, e) of the staging
 // array: the number of the segment's cells ordered before it (lexicographic, equal
 // cells in staging order).  One thread per cell, so a vertex with many cells costs
-// its segment length per cell instead of a quadratic sort in one lane.
+// its segment length per cell instead of a quadratic sort in one lane; with step > 1
+// the count over cells s, s + step, ... (the caller sums the lanes' parts).
 template <int NV>
-__device__ __forceinline__ uint32_t cell_rank(const uint32_t* stage, uint32_t s, uint32_t e, uint32_t c) {
+__device__ __forceinline__ uint32_t cell_rank(const uint32_t* stage, uint32_t s, uint32_t e, uint32_t c,
+                                              uint32_t step = 1) {
   uint32_t r = 0;
   if constexpr (NV == 2) {
     const uint64_t kc = ((uint64_t)stage[2 * c] << 32) | stage[2 * c + 1];
@@ -1649,7 +1651,7 @@ __device__ __forceinline__ uint32_t cell_rank(const uint32_t* stage, uint32_t s,
     };
     const uint64_t ch = hi(c), cl = lo(c);
 #pragma unroll 4
-    for (uint32_t j = s; j < e; ++j) {
+    for (uint32_t j = s; j < e; j += step) {
       const uint64_t jh = hi(j), jl = lo(j);
       r += (jh < ch || (jh == ch && (jl < cl || (jl == cl && j < c)))) ? 1u : 0u;
     }
@@ -1935,7 +1937,14 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
       }
       // each cell goes straight to its sorted place in the workgroup's output range
       auto dst = gptr(P.cells) + cbase * NV;
-      for (uint32_t c = threadIdx.x; c < ctot; c += BLOCK) {
+      // Cells of three or four nodes come few per workgroup but many per vertex
+      // (up to 72 per hypercube for marching cells): four consecutive lanes count
+      // each one's rank over every fourth cell of its segment, so that the LDS
+      // reads of a long segment are spread over more lanes.  Lane groups are whole
+      // (ctot * G and BLOCK are multiples of G), so the shuffles see all four.
+      constexpr uint32_t G = NV >= 3 ? 4 : 1;
+      for (uint32_t w = threadIdx.x; w < ctot * G; w += BLOCK) {
+        const uint32_t c = w / G, part = w % G;
         uint32_t lo;
         if constexpr (OWNERS) {
           lo = cown[c];
@@ -1951,9 +1960,16 @@ __device__ __forceinline__ void emit_tile(const KParams& P, int64_t blk) {
           }
         }
         const uint32_t sb = segx[lo];
-        const uint32_t pos = sb + cell_rank<NV>(stage, sb, segx[lo + 1], c);
+        uint32_t r = cell_rank<NV>(stage, sb + part, segx[lo + 1], c, G);
+        if constexpr (G > 1) {
+          r += __shfl_xor(r, 1);
+          r += __shfl_xor(r, 2);
+        }
+        if (part == 0) {
+          const uint32_t pos = sb + r;
 #pragma unroll
-        for (int q = 0; q < NV; ++q) dst[pos * NV + q] = stage[c * NV + q];
+          for (int q = 0; q < NV; ++q) dst[pos * NV + q] = stage[c * NV + q];
+        }
       }
       // Union the cells whose nodes all belong to this workgroup in LDS and start
       // the global union-find from those roots (roots stay component minima).

@@ -189,6 +189,11 @@ def test_emit_overflow_paths(monkeypatch):
     compare("cp3d_emit_overflow", comps, "cp", 3)
     s = synth.scalar_config(n=20, nt=4, ngauss=10, seed=2, dtype=np.float32)
     compare("contour_emit_overflow", [s], "contour", 1)
+    a = synth.scalar_config(n=20, nt=4, ngauss=10, seed=5, dtype=np.float32)
+    b = synth.scalar_config(n=20, nt=4, ngauss=10, seed=6, dtype=np.float32)
+    compare("fiber_emit_overflow", [a, b], "fiber", 2)
+    t = synth.gaussians((26, 24), 5, 6, gradient=False, seed=8, subtract_mean=True)[0]
+    compare("contour_2d+t_emit_overflow", [t], "contour", 1)
 
 
 def test_fiber3d_reference_case():
