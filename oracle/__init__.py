@@ -113,6 +113,18 @@ def _collect(h):
         L.oracle_free(h)
 
 
+def _dtype_code(arrays):
+    """0 for float32, 1 for float64; every array must share it (the C side reads all
+    of them with the first one's element size)."""
+    dt = arrays[0].dtype
+    if dt not in (np.float32, np.float64):
+        raise TypeError(f"field arrays must be float32 or float64, not {dt}")
+    for a in arrays:
+        if a.dtype != dt:
+            raise TypeError(f"field arrays mix {dt} and {a.dtype}")
+    return 0 if dt == np.float32 else 1
+
+
 def run(comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e6, scalar=None,
         nthreads=0, time_stride=0):
     """Run the CPU engine on a regular mesh.
@@ -123,7 +135,7 @@ def run(comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e
     id = spatial id + t * time_stride, see extruded_time_stride).
     """
     comps = list(comps)
-    dtype = 0 if comps[0].dtype == np.float32 else 1
+    dtype = _dtype_code(comps)
     d = comps[0].ndim
     if dims is None:
         dims = comps[0].shape
@@ -157,7 +169,7 @@ class Lean:
 
     def __init__(self, comps, kind, m, dims=None, offset=None, gdims=None, thr=(0.0, 0.0), Q=1e6, nthreads=0):
         comps = list(comps)
-        dtype = 0 if comps[0].dtype == np.float32 else 1
+        dtype = _dtype_code(comps)
         d = comps[0].ndim
         dims = comps[0].shape if dims is None else dims
         item = comps[0].itemsize
@@ -223,7 +235,7 @@ def run_stream(steps, kind, m, thr=(0.0, 0.0), Q=1e6, nthreads=0):
     nt = len(steps)
     ncomp = len(steps[0])
     sp = steps[0][0].shape
-    dtype = 0 if steps[0][0].dtype == np.float32 else 1
+    dtype = _dtype_code([a for st in steps for a in st])
     flat = []
     for t in range(nt):
         for c in range(ncomp):

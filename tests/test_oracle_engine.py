@@ -171,3 +171,19 @@ def test_stream_equals_oneshot_every_predicate(case):
     assert sorted(one["conn"]) == sorted(st["conn"])
     for d in one["conn"]:
         assert np.array_equal(one["conn"][d], st["conn"][d])
+
+
+def test_field_arrays_must_share_one_dtype():
+    """The C side reads every component with the first one's element size, so the
+    binding refuses components of mixed dtypes (a float32 array read as float64
+    would be read past its end) and dtypes other than float32 / float64."""
+    a = np.zeros((6, 5, 3), np.float32)
+    b = np.zeros((6, 5, 3), np.float64)
+    with pytest.raises(TypeError):
+        oracle.run([a, b], oracle.PRED_FIBER, 2)
+    steps = [[np.asfortranarray(a[..., t]), np.asfortranarray(b[..., t])] for t in range(3)]
+    with pytest.raises(TypeError):
+        oracle.run_stream(steps, oracle.PRED_FIBER, 2)
+    with pytest.raises(TypeError):
+        oracle.run([np.zeros((6, 5, 3), np.int32)], oracle.PRED_CONTOUR, 1)
+    oracle.run([a, a.copy()], oracle.PRED_FIBER, 2)  # one dtype: accepted
