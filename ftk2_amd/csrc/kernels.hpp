@@ -1273,6 +1273,46 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
         if (inside3_exact_d(X, mx)) atomicOr(&prom[owner], 1ull << t);
       });
     mk.sol |= prom[threadIdx.x];
+    if constexpr (KIND == KIND_CP) {
+      // A tile whose pending simplices all wait for the solver gate alone (no
+      // verbatim sign levels left) finishes here, as stage 2 would finish it: the
+      // tile of quantized corners is read no more, so its LDS holds the owners'
+      // references, the accepted bits and the corner tables; the certificate
+      // accepts what it can (FP64), the solver decides the rest.  Stage 2 then
+      // skips the tile on its count.
+      if (__syncthreads_or(mk.sol != 0) && !__syncthreads_or(mk.deg != 0)) {
+        static_assert(BLOCK * (sizeof(VtxRef) + 8) + sizeof(CornerTab) <= sizeof(tile), "gate state does not fit");
+        VtxRef* refs = reinterpret_cast<VtxRef*>(tile);
+        unsigned long long* slot = reinterpret_cast<unsigned long long*>(refs + BLOCK);
+        CornerTab* tab = reinterpret_cast<CornerTab*>(slot + BLOCK);
+        load_corner_tab<D, M>(P, tab);
+        if (mk.sol) refs[threadIdx.x] = make_vtx<D>(P, v).r;
+        slot[threadIdx.x] = mk.acc;
+        prom[threadIdx.x] = 0;
+        uint64_t ws = mk.sol;
+        if constexpr (std::is_same<T, double>::value) {
+          run_items(il, ws, [&](int owner, int t) {
+            T vals[M + 1][M];
+            item_values<KIND, D, M, T>(P, *tab, refs[owner], t, vals);
+            if (solver_certified<M>(vals))
+              atomicOr(&slot[owner], 1ull << t);
+            else
+              atomicOr(&prom[owner], 1ull << t);
+          });
+          ws = prom[threadIdx.x];
+        }
+        if (__syncthreads_or(ws != 0))
+          run_items(il, ws, [&](int owner, int t) {
+            if (item_solves<KIND, D, M, T>(P, *tab, refs[owner], t)) atomicOr(&slot[owner], 1ull << t);
+          });
+        const uint64_t acc = slot[threadIdx.x];
+        if (v < P.nvert) gptr(P.mask)[v] = acc;
+        uint32_t tot;
+        block_excl_scan((uint32_t)__popcll(acc), &tot);
+        if (threadIdx.x == 0) gptr(P.bsum)[blk] = tot;
+        return;
+      }
+    }
   }
   // A tile that leaves nothing to stage 2 has its node count now: it goes to bsum,
   // and stage 2 skips the tile.  Other tiles get a TILE_PENDING sentinel until
