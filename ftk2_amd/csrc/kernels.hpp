@@ -623,8 +623,22 @@ __device__ __forceinline__ Masks classify_tri(const KParams& P, const Vtx& x, co
 
 // Edge predicates: ContourPredicate<T> (sign change of value - threshold) and
 // CriticalPointPredicate<1,T> (sign change + solver gate).
+// The sign class of one corner value for the edge predicates, staged once per
+// workgroup (stage_sign_classes): 1 / -1 when the quantized value is at least one
+// quantum from zero, 0 for a tie, whose sos_sign comes from the corner's global id.
+template <int KIND, class T>
+__device__ __forceinline__ int8_t sign_class(const KParams& P, T v) {
+  if constexpr (KIND == KIND_CONTOUR) v = v - (T)P.thr[0];
+  const double r = qd(v, P.Q);  // as sos_sign
+  return r >= 1.0 ? (int8_t)1 : (r <= -1.0 ? (int8_t)-1 : (int8_t)0);
+}
 template <int KIND, class T, int D>
-__device__ __forceinline__ Masks classify_edge(const KParams& P, const Vtx& x) {
+__device__ __forceinline__ void stage_sign_classes(const KParams& P, int64_t v0, int8_t* cls) {
+  stage_tile<T, D, 1>(P, v0, [&](int e, const T* v) { cls[e] = sign_class<KIND, T>(P, v[0]); });
+}
+
+template <int KIND, class T, int D>
+__device__ __forceinline__ Masks classify_edge(const KParams& P, const Vtx& x, const int8_t* cls) {
   uint64_t acc = 0, sol = 0, deg = 0;
   constexpr int NC = 1 << D;
   const uint32_t inb = corner_mask<D>(P, x.l);
@@ -632,9 +646,9 @@ __device__ __forceinline__ Masks classify_edge(const KParams& P, const Vtx& x) {
 #pragma unroll
   for (int s = 0; s < NC; ++s) {
     const bool in = (inb >> s) & 1;
-    T v = load_val<T>(P, 0, x.r, in ? s : 0);
-    if constexpr (KIND == KIND_CONTOUR) v = v - (T)P.thr[0];
-    sg[s] = in ? sos_sign(v, x.r.gid + P.goff[s], P.Q) : 1;
+    const int c = cls[(s >> 1) * Tile3<D>::SEG + threadIdx.x + (s & 1)];
+    const int tie = ((x.r.gid + P.goff[s]) % 2 == 0) ? 1 : -1;
+    sg[s] = in ? (c != 0 ? c : tie) : 1;
   }
 #pragma unroll
   for (int s = 1; s < NC; ++s) {
@@ -655,7 +669,7 @@ __device__ __forceinline__ Masks classify_fast(const KParams& P, const Vtx& x, c
   else if constexpr (M == 2)
     return classify_tri<KIND, T, D>(P, x, reinterpret_cast<const double*>(tile));
   else
-    return classify_edge<KIND, T, D>(P, x);
+    return classify_edge<KIND, T, D>(P, x, reinterpret_cast<const int8_t*>(tile));
 }
 
 // Stage 2 helpers for one pending simplex (type t owned by the vertex of r).
@@ -1254,7 +1268,9 @@ __device__ __forceinline__ bool gate_in_stage1(const KParams& P, int64_t blk, in
 // verbatim sign levels), degm = the verbatim ones among them.
 template <int KIND, int D, int M, class T>
 __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
-  constexpr int TBYTES = M == 3 ? Tile3<D>::FLOATS * 4 : (TILE2<D, M> ? Tile3<D>::NSEG * Tile3<D>::SEG * 16 : 16);
+  constexpr int TBYTES = M == 3   ? Tile3<D>::FLOATS * 4
+                         : M == 1 ? (Tile3<D>::NSEG * Tile3<D>::SEG + 15) / 16 * 16
+                                  : (TILE2<D, M> ? Tile3<D>::NSEG * Tile3<D>::SEG * 16 : 16);
   __shared__ __align__(16) float tile[TBYTES / 4];
   const int64_t v0 = blk * BLOCK;
   const int64_t v = v0 + threadIdx.x;
@@ -1263,6 +1279,9 @@ __device__ __forceinline__ void classify_tile(const KParams& P, int64_t blk) {
     __syncthreads();
   } else if constexpr (TILE2<D, M>) {
     stage_quantized2<KIND, T, D>(P, v0, reinterpret_cast<double*>(tile));
+    __syncthreads();
+  } else if constexpr (M == 1) {
+    stage_sign_classes<KIND, T, D>(P, v0, reinterpret_cast<int8_t*>(tile));
     __syncthreads();
   }
   Masks mk{0, 0, 0, 0};
